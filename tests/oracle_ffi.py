@@ -1,0 +1,150 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so) and to the
+reference's own GSL-free sources compiled by oracle/build_ref.sh. Test
+infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libqpms_ref.so")
+
+vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+class c_cplx(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, z):
+        z = complex(z)
+        return cls(z.real, z.imag)
+
+
+_o = None
+
+
+def oracle():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = ctypes.CDLL(ORACLE_SO)
+        L.or_calc_init.restype = vp
+        L.or_calc_init.argtypes = [i32, i32]
+        L.or_calc_free.argtypes = [vp]
+        L.or_get_AB_arrays.argtypes = [vp, c_cplx, dbl, dbl, i32, i32, vp, vp]
+        L.or_trans_single_A.restype = c_cplx
+        L.or_trans_single_A.argtypes = [i32, i32, i32, i32, i32, c_cplx, dbl, dbl, i32]
+        L.or_trans_single_B.restype = c_cplx
+        L.or_trans_single_B.argtypes = [i32, i32, i32, i32, i32, c_cplx, dbl, dbl, i32]
+        L.or_build_W_rows.argtypes = [vp, i32, vp, c_cplx, i32, i32, i32, vp]
+        L.or_build_M.argtypes = [vp, i32, vp, vp, vp, c_cplx, vp]
+        L.or_mie.argtypes = [i32, dbl, c_cplx, c_cplx, c_cplx, c_cplx, vp, vp]
+        L.or_zgetrf.argtypes = [i32, vp, vp, i32]
+        L.or_zgetrf.restype = i32
+        L.or_zgetrs.argtypes = [i32, i32, vp, vp, vp]
+        L.or_gaunt.restype = dbl
+        L.or_gaunt.argtypes = [i32] * 5
+        L.or_wigner3j.restype = dbl
+        L.or_wigner3j.argtypes = [i32] * 6
+        L.or_legendre_array.argtypes = [i32, dbl, i32, vp]
+        L.or_sph_bessel.argtypes = [i32, i32, c_cplx, vp]
+        L.or_calc_multipliers.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.or_set_threads.argtypes = [i32]
+        _o = L
+    return _o
+
+
+def ref():
+    """Reference gaunt.c / latticegens.c / lattices2d.c, or None if not built."""
+    if not os.path.exists(REF_SO):
+        return None
+    L = ctypes.CDLL(REF_SO, mode=os.RTLD_LAZY)
+    L.gaunt_xu.argtypes = [i32, i32, i32, i32, i32, vp, vp]
+    return L
+
+
+class OracleCalc:
+    def __init__(self, lMax, norm=65536 | 8):
+        self.o = oracle()
+        self.c = self.o.or_calc_init(lMax, norm)
+        self.lMax, self.nelem, self.norm = lMax, lMax * (lMax + 2), norm
+
+    def __del__(self):
+        try:
+            self.o.or_calc_free(self.c)
+        except Exception:
+            pass
+
+    def get_AB_arrays(self, kr, theta, phi, J=3, r_ge_d=False):
+        ne = self.nelem
+        A = np.empty((ne, ne), np.complex128)
+        B = np.empty((ne, ne), np.complex128)
+        self.o.or_get_AB_arrays(self.c, c_cplx.of(kr), theta, phi, int(r_ge_d), J, A.ctypes.data, B.ctypes.data)
+        return A, B
+
+    def multipliers(self, y, yu):
+        A = np.zeros(self.lMax + 1, np.complex128)
+        B = np.zeros(self.lMax + 1, np.complex128)
+        nA, nB = ctypes.c_int(), ctypes.c_int()
+        self.o.or_calc_multipliers(self.c, y, yu, A.ctypes.data, ctypes.byref(nA), B.ctypes.data, ctypes.byref(nB))
+        return A[:nA.value], B[:nB.value]
+
+    def build_W_rows(self, pos, k, row0, nrows, J=3):
+        pos = np.ascontiguousarray(pos, np.float64)
+        npart = pos.shape[0]
+        nb = 2 * self.nelem
+        W = np.empty((nrows * nb, npart * nb), np.complex128)
+        self.o.or_build_W_rows(self.c, npart, pos.ctypes.data, c_cplx.of(k), J, row0, nrows, W.ctypes.data)
+        return W
+
+    def build_M(self, pos, ttype, tdiag, k):
+        pos = np.ascontiguousarray(pos, np.float64)
+        npart = pos.shape[0]
+        L = npart * 2 * self.nelem
+        M = np.empty((L, L), np.complex128)
+        ttype = np.ascontiguousarray(ttype, np.int32)
+        tdiag = np.ascontiguousarray(tdiag, np.complex128)
+        self.o.or_build_M(self.c, npart, pos.ctypes.data, ttype.ctypes.data, tdiag.ctypes.data, c_cplx.of(k),
+                          M.ctypes.data)
+        return M
+
+
+def sph_bessel(J, lmax, x):
+    out = np.empty(lmax + 1, np.complex128)
+    oracle().or_sph_bessel(J, lmax, c_cplx.of(x), out.ctypes.data)
+    return out
+
+
+def legendre_array(lmax, x, csphase=-1):
+    out = np.empty((lmax + 1) * (lmax + 2) // 2)
+    oracle().or_legendre_array(lmax, x, csphase, out.ctypes.data)
+    return out
+
+
+def mie(lMax, a, ki, ke, mui=1, mue=1):
+    RV = np.empty(lMax + 1, np.complex128)
+    RH = np.empty(lMax + 1, np.complex128)
+    oracle().or_mie(lMax, a, c_cplx.of(ki), c_cplx.of(ke), c_cplx.of(mui), c_cplx.of(mue), RV.ctypes.data,
+                    RH.ctypes.data)
+    return RV, RH
+
+
+def zgetrf(A, nthreads=1):
+    A = np.ascontiguousarray(A, np.complex128).copy()
+    n = A.shape[0]
+    ipiv = np.empty(n, np.int32)
+    info = oracle().or_zgetrf(n, A.ctypes.data, ipiv.ctypes.data, nthreads)
+    return A, ipiv, info
+
+
+def zgetrs(LU, ipiv, B):
+    B = np.ascontiguousarray(B, np.complex128).copy()
+    n = LU.shape[0]
+    nrhs = 1 if B.ndim == 1 else B.shape[1]
+    oracle().or_zgetrs(n, nrhs, np.ascontiguousarray(LU).ctypes.data, np.ascontiguousarray(ipiv, np.int32).ctypes.data,
+                       B.ctypes.data)
+    return B
