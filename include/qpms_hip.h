@@ -560,6 +560,8 @@ int qpms_hip_init(void);
 const char *qpms_hip_version(void);
 /* Synchronise the library's stream. */
 int qpms_hip_sync(void);
+/* The library's own hipStream_t (the one NULL stream arguments select). */
+void *qpms_hip_stream(void);
 /* Device memory helpers for the *_dev entry points (bytes). */
 void *qpms_hip_malloc(size_t bytes);
 void qpms_hip_free(void *dptr);
@@ -601,6 +603,16 @@ int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_dev, void *s
 /* Solve with nrhs right-hand sides stored row-major (n × nrhs). */
 int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, const int *ipiv_dev,
 		_Complex double *B_dev, void *stream);
+
+/* Batched Ewald sums: ntrip (beta, k, s) triples on one 2D lattice in the xy
+ * plane. triples_dev holds 8 doubles per triple {beta_x, beta_y, beta_z,
+ * Re k, Im k, s_x, s_y, s_z}; R_dev nR direct-lattice points (x, y, z), the
+ * triple's s subtracted in-kernel; K_dev nK reciprocal points (x, y).
+ * Outputs (either may be NULL) hold c->nelem_sc complex values per triple,
+ * the same quantities as ewald3_sigma_short / ewald3_sigma_long. */
+int qpms_hip_ewald3_sigma_batch_dev(const qpms_ewald3_constants_t *c, double eta, double unitcell_area,
+		size_t ntrip, const double *triples_dev, size_t nR, const double *R_dev, size_t nK, const double *K_dev,
+		_Complex double *sigma_short_dev, _Complex double *sigma_long_dev, void *stream);
 
 /* Mixed-precision solve: X = A⁻¹ B with an FP32-complex LU on MFMA and FP64
  * iterative refinement until ‖B − A X‖ / (‖A‖‖X‖) ≤ tol or maxit steps.

@@ -73,7 +73,6 @@ void or_zgetrs(int n, int nrhs, const double complex *LU, const int *ipiv, doubl
 /* Ewald (ewald.c): xyWeb point enumeration and the σ sums. */
 int or_xyweb_points(double b1x, double b1y, double b2x, double b2y, double rtol, double minR, int inc_minR,
 		double maxR, int inc_maxR, double *xy, int cap);
-void or_ewald_constants(int lMax_e, int csphase, double complex *s1, int *s1_off, double *leg0);
 int or_sigma0(int lMax_e, double eta, double complex k, double complex *out);
 int or_sigma_short(int lMax_e, int csphase, double eta, double complex k, int npts, const double *Rxy,
 		double betax, double betay, double sx, double sy, double complex *out);

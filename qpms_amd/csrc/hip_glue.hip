@@ -58,6 +58,7 @@ extern "C" void qh_memset(void *dst, int v, size_t bytes, void *s) {
 extern "C" int qpms_hip_init(void) { return qh_dev_init(); }
 extern "C" const char *qpms_hip_version(void) { return "qpms_hip 0.1 (gfx950)"; }
 extern "C" int qpms_hip_sync(void) { qh_sync(nullptr); return 0; }
+extern "C" void *qpms_hip_stream(void) { return qh_stream(nullptr); }
 extern "C" void *qpms_hip_malloc(size_t bytes) { return qh_dmalloc(bytes); }
 extern "C" void qpms_hip_free(void *p) { qh_dfree(p); }
 extern "C" int qpms_hip_memcpy_h2d(void *dst, const void *src, size_t bytes) {

@@ -1,0 +1,388 @@
+// 2D-periodic Ewald lattice sums on gfx950 (ewald.c:260-920) and the e32
+// combination into translation coefficients (translations.c:926-1094).
+//
+// One wave64 per (beta, k, s) "triple": lanes stride over the lattice points
+// (R for the short-range part, K for the long-range part), each lane keeps the
+// sigma_{n,m} partial sums in registers, and a shuffle reduction finishes the
+// triple. The point lists are shared by all triples of a batch (one lattice),
+// so they stream from L1/L2. All arithmetic is FP64; the short-range
+// integrals use the Faddeeva closed form and the long-range part
+// Gamma(1/2 - j, x^2) is evaluated in-kernel (dev_ewald.hpp).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <mutex>
+#include "dev_ewald.hpp"
+#include "dev_specfunc.hpp"
+#include "qpms_internal.h"
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+  qpms_hip_die(__FILE__, __LINE__, __func__, "HIP error %d (%s) in %s", (int)e_, hipGetErrorString(e_), #x); } while (0)
+
+__constant__ double c_weid[qew::WEID_N];
+
+namespace {
+
+constexpr int cabsi(int a) { return a < 0 ? -a : a; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Short range (ewald.c:742-920):
+//   sigma^S_{n,m} = sum_R -i (2/k)^{n+1}/sqrt(pi) |R-s|^n P_n^{|m|}(cos th)
+//                   I_n(|R-s|) e^{i beta.(R-s)} e^{i m phi} (-1)^m [m < 0]
+// pts: lattice points (x, y, z); the triple's shift s is subtracted.
+// XY: all points and s in the plane z = 0, P_n^{|m|}(0) from leg0 and the
+// odd n+m coefficients vanish.
+// ---------------------------------------------------------------------------
+template <int LE, bool XY>
+__global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__restrict__ tr, int npts,
+                                                     const double *__restrict__ pts, double eta,
+                                                     const double *__restrict__ leg0, int csphase,
+                                                     v2d *__restrict__ out, double *__restrict__ err) {
+  constexpr int NSC = (LE + 1) * (LE + 1);
+  const int t = blockIdx.x;
+  if (t >= ntrip) return;
+  const int lane = threadIdx.x;
+  const double *T = tr + 8 * (size_t)t;  // bx by bz kre kim sx sy sz
+  const cplx k = mk(T[3], T[4]);
+  const bool kreal = (T[4] == 0.0);
+  cplx pre[LE + 1];  // -i (2/k)^{n+1} / sqrt(pi)
+  {
+    const cplx tk = kreal ? mk(2.0 / T[3], 0.0) : 2.0 * cinv(k);
+    cplx p = tk;
+#pragma unroll
+    for (int n = 0; n <= LE; ++n) {
+      pre[n] = mk(p.im, -p.re) * 0.56418958354775628695;
+      p = p * tk;
+    }
+  }
+  cplx acc[NSC];
+#pragma unroll
+  for (int i = 0; i < NSC; ++i) acc[i] = mk(0.0, 0.0);
+  double aerr = 0.0;
+  for (int ip = lane; ip < npts; ip += 64) {
+    const double rx = pts[3 * ip] - T[5], ry = pts[3 * ip + 1] - T[6], rz = pts[3 * ip + 2] - T[7];
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    cplx In[LE + 1];
+    qew::sr_integrals<LE>(r, k, eta, c_weid, In);
+    const cplx eb = cexpi(T[0] * rx + T[1] * ry + T[2] * rz);
+    const cplx e1 = cexpi(atan2(ry, rx));
+    double P[XY ? 1 : qsf::leg_n(LE)];
+    if (!XY) {
+      const double x = rz / r;
+      qsf::legendre_cs<LE>(x, sqrt((1.0 - x) * (1.0 + x)), P);
+    }
+    cplx em[LE + 1];
+    em[0] = mk(1.0, 0.0);
+#pragma unroll
+    for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
+    double rn = 1.0;
+#pragma unroll
+    for (int n = 0; n <= LE; ++n) {
+      const cplx base = (pre[n] * (rn * In[n])) * eb;
+#pragma unroll
+      for (int m = -n; m <= n; ++m) {
+        if (XY && ((n + m) & 1)) continue;
+        const int am = cabsi(m);
+        double leg;
+        if (XY) leg = leg0[n * (n + 1) / 2 + am];
+        else leg = ((csphase == 1) && (am & 1)) ? -P[n * (n + 1) / 2 + am] : P[n * (n + 1) / 2 + am];
+        const cplx e = (m >= 0) ? em[am] : conj(em[am]);
+        cplx s = (leg * base) * e;
+        if (m < 0 && (am & 1)) s = -s;
+        acc[n * (n + 1) + m] = acc[n * (n + 1) + m] + s;
+        aerr += fabs(s.re) + fabs(s.im);
+      }
+      rn *= r;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NSC; ++i) {
+    const double re = wave_sum(acc[i].re), im = wave_sum(acc[i].im);
+    if (lane == 0) out[(size_t)t * NSC + i] = v2d{re, im};
+  }
+  aerr = wave_sum(aerr);
+  if (err && lane == 0)
+    for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr;
+}
+
+// ---------------------------------------------------------------------------
+// Long range, lattice in the xy plane, in-plane shift (ewald.c:286-503):
+//   sigma^L_{n,m} = 1/(k^2 A) sum_K e^{i K.s} e^{i m alpha} P_n^{|m|}(0) (-1)^m[m<0]
+//     sum_j (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, x^2) c_{n,m,j},
+//   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
+// s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
+// ---------------------------------------------------------------------------
+template <int LE>
+__global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__restrict__ tr, int npts,
+                                                    const double *__restrict__ Kp /* x, y */, double eta,
+                                                    double area, const double *__restrict__ leg0,
+                                                    const v2d *__restrict__ s1, const int *__restrict__ s1off,
+                                                    v2d *__restrict__ out, double *__restrict__ err) {
+  constexpr int NSC = (LE + 1) * (LE + 1);
+  constexpr int JM = LE / 2;
+  const int t = blockIdx.x;
+  if (t >= ntrip) return;
+  const int lane = threadIdx.x;
+  const double *T = tr + 8 * (size_t)t;
+  const cplx k = mk(T[3], T[4]);
+  const bool kreal = (T[4] == 0.0);
+  cplx acc[NSC];
+#pragma unroll
+  for (int i = 0; i < NSC; ++i) acc[i] = mk(0.0, 0.0);
+  double aerr = 0.0;
+  for (int ip = lane; ip < npts; ip += 64) {
+    const double Kx = Kp[2 * ip], Ky = Kp[2 * ip + 1];
+    const double bx = Kx + T[0], by = Ky + T[1];
+    const double rb = sqrt(bx * bx + by * by);
+    const double alpha = atan2(by, bx);
+    // gamma, x^2 and the incomplete-gamma branch (ewald.c:384-396)
+    cplx gam, z;
+    int branch;
+    if (kreal) {
+      const double tt = rb / k.re;
+      if (tt >= 1.0) {
+        gam = mk(sqrt(tt - 1.0) * sqrt(tt + 1.0), 0.0);
+        const double x = gam.re * k.re / (2.0 * eta);
+        z = mk(x * x, 0.0);
+      } else {
+        gam = mk(0.0, -(sqrt(1.0 - tt) * sqrt(1.0 + tt)));
+        const double x = gam.im * k.re / (2.0 * eta);
+        z = mk(-(x * x), -0.0);  // (i x)^2 lies just below the cut
+      }
+      branch = 0;
+    } else {
+      gam = qew::clilgamma(mk(rb, 0.0) / k);
+      const cplx x = (gam * k) / (2.0 * eta);
+      z = x * x;
+      branch = (z.re < 0 && !signbit(z.im)) ? -1 : 0;
+    }
+    if (z.re > 740.0) continue;  // every Gamma(1/2-j, z) underflows
+    cplx G[JM + 1];
+#pragma unroll
+    for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
+    // gamma^{2j-1}
+    cplx gp[JM + 1];
+    gp[0] = cinv(gam);
+    const cplx g2 = gam * gam;
+#pragma unroll
+    for (int j = 1; j <= JM; ++j) gp[j] = (j == 1) ? gam : gp[j - 1] * g2;
+    // (|beta+K|/k)^p, p = 0..LE
+    const cplx rk = kreal ? mk(rb / k.re, 0.0) : mk(rb, 0.0) / k;
+    cplx rkp[LE + 1];
+    rkp[0] = mk(1.0, 0.0);
+#pragma unroll
+    for (int p = 1; p <= LE; ++p) rkp[p] = rkp[p - 1] * rk;
+    const cplx phase = cexpi(Kx * T[5] + Ky * T[6]);
+    const cplx e1 = cexpi(alpha);
+    cplx em[LE + 1];
+    em[0] = mk(1.0, 0.0);
+#pragma unroll
+    for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
+#pragma unroll
+    for (int n = 0; n <= LE; ++n) {
+#pragma unroll
+      for (int m = -n; m <= n; ++m) {
+        if ((n + m) & 1) continue;
+        const int am = cabsi(m);
+        const int y = n * (n + 1) + m;
+        cplx js = mk(0.0, 0.0);
+        const int off = s1off[y];
+#pragma unroll
+        for (int j = 0; j <= (n - am) / 2; ++j) {
+          const v2d c = s1[off + j];
+          js = js + ((rkp[n - 2 * j] * gp[j]) * G[j]) * mk(c.x, c.y);
+        }
+        const cplx e = (m >= 0) ? em[am] : conj(em[am]);
+        cplx s = (leg0[n * (n + 1) / 2 + am] * js) * e * phase;
+        if (m < 0 && (am & 1)) s = -s;
+        acc[y] = acc[y] + s;
+        aerr += fabs(s.re) + fabs(s.im);
+      }
+    }
+  }
+  const cplx cf = kreal ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
+#pragma unroll
+  for (int i = 0; i < NSC; ++i) {
+    const double re = wave_sum(acc[i].re), im = wave_sum(acc[i].im);
+    if (lane == 0) {
+      const cplx v = mk(re, im) * cf;
+      out[(size_t)t * NSC + i] = v2d{v.re, v.im};
+    }
+  }
+  aerr = wave_sum(aerr);
+  if (err && lane == 0)
+    for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr * cabsd(cf);
+}
+
+// sigma0 = Gamma(-1/2, -(k/2 eta)^2) P_0^0(0) / (2 sqrt(pi)) (ewald.c:260-273)
+__global__ void k_sigma0(int n, const double *__restrict__ kk, double eta, double leg00, v2d *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx k = mk(kk[2 * i], kk[2 * i + 1]);
+  cplx z;
+  if (k.im == 0.0) {
+    const double h = k.re / (2.0 * eta);
+    z = mk(-(h * h), -0.0);  // -(k/2eta)^2 with Im = -0 for real k
+  } else {
+    const cplx h = k / (2.0 * eta);
+    z = -(h * h);
+  }
+  const int branch = (z.re < 0 && !signbit(z.im)) ? -1 : 0;
+  const cplx g = qew::gamma_inc_half(1, z, branch);
+  const cplx v = g * (leg00 / 2.0 / 1.7724538509055160273);
+  out[i] = v2d{v.re, v.im};
+}
+
+// ---------------------------------------------------------------------------
+// e32 combine (translations.c:1024-1067): per triple, from sigma_total
+//   A(y<-yu) = sum_{q<qA} mult_A[q] sigma[y_sc(mu-m, n+nu-2q)]
+//   B(y<-yu) = sum_{q<qB'} mult_B[q] sigma[y_sc(mu-m, n+nu-2q+1)]
+// qB' = qA (bq_mode 0, the reference's loop bound) or B's own count (mode 1).
+// ---------------------------------------------------------------------------
+__global__ void k_e32_combine(int ntrip, int nelem, int qslots, int nsc, const v2d *__restrict__ sig,
+                              const v2d *__restrict__ mA, const v2d *__restrict__ mB,
+                              const unsigned char *__restrict__ qA, const unsigned char *__restrict__ qB,
+                              int bq_mode, v2d *__restrict__ A, v2d *__restrict__ B) {
+  const int t = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrip || e >= nelem * nelem) return;
+  const int y = e / nelem, yu = e % nelem;
+  const int n = (int)sqrt((double)y + 1), m = y - n * (n + 1) + 1;
+  const int nu = (int)sqrt((double)yu + 1), mu = yu - nu * (nu + 1) + 1;
+  const v2d *s = sig + (size_t)t * nsc;
+  const int na = qA[e], nb = bq_mode ? qB[e] : qA[e];
+  cplx sa = mk(0, 0), sb = mk(0, 0);
+  for (int q = 0; q < na; ++q) {
+    const int p = n + nu - 2 * q;
+    const v2d mq = mA[(size_t)e * qslots + q], sv = s[p * (p + 1) + mu - m];
+    cfma(sa, mk(mq.x, mq.y), mk(sv.x, sv.y));
+  }
+  for (int q = 0; q < nb; ++q) {
+    const int p = n + nu - 2 * q + 1;
+    const v2d mq = mB[(size_t)e * qslots + q], sv = s[p * (p + 1) + mu - m];
+    cfma(sb, mk(mq.x, mq.y), mk(sv.x, sv.y));
+  }
+  A[(size_t)t * nelem * nelem + e] = v2d{sa.re, sa.im};
+  B[(size_t)t * nelem * nelem + e] = v2d{sb.re, sb.im};
+}
+
+// sigma_total = short + long (+ sigma0 on y = 0 when flag0[t])
+__global__ void k_sigma_total(int ntrip, int nsc, const v2d *__restrict__ ss, const v2d *__restrict__ sl,
+                              const v2d *__restrict__ s0, const int *__restrict__ flag0, int parts,
+                              v2d *__restrict__ tot) {
+  const int t = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrip || i >= nsc) return;
+  v2d v = v2d{0, 0};
+  if (parts & 2) v += ss[(size_t)t * nsc + i];
+  if (parts & 1) v += sl[(size_t)t * nsc + i];
+  if ((parts & 4) && i == 0 && flag0[t]) v += s0[t];
+  tot[(size_t)t * nsc + i] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Weideman coefficients (host): a_m, m = 1..N, of Weideman's eq. (3.6), from
+// the discrete Fourier sum over t_k = L tan(theta_k / 2), theta_k = k pi / M.
+// ---------------------------------------------------------------------------
+static void weideman_coefficients(double *a) {
+  const int N = qew::WEID_N, M = 2 * N, M2 = 2 * M;
+  const double L = sqrt(N / sqrt(2.0));
+  double f[2 * 2 * qew::WEID_N];
+  f[0] = 0.0;
+  for (int k = -M + 1; k <= M - 1; ++k) {
+    const double th = k * M_PI / M, tt = L * tan(th / 2);
+    f[k + M] = exp(-tt * tt) * (L * L + tt * tt);
+  }
+  double g[2 * 2 * qew::WEID_N];
+  for (int i = 0; i < M2; ++i) g[i] = f[(i + M) % M2];  // fftshift
+  for (int m = 1; m <= N; ++m) {
+    long double s = 0;
+    for (int i = 0; i < M2; ++i) s += (long double)g[i] * cosl(2.0L * M_PI * (long double)i * m / M2);
+    a[m - 1] = (double)(s / M2);
+  }
+}
+
+static std::once_flag g_weid_once;
+static void ensure_weid() {
+  std::call_once(g_weid_once, [] {
+    double a[qew::WEID_N];
+    weideman_coefficients(a);
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_weid), a, sizeof a, 0, hipMemcpyHostToDevice));
+  });
+}
+
+extern "C" void qh_weideman_coefficients(double *a) { weideman_coefficients(a); }
+
+extern "C" int qh_launch_sigma_short(int LE, int xy, int ntrip, const double *tr_dev, int npts, const double *pts_dev,
+                                     double eta, const double *leg0_dev, int csphase, double *out_dev,
+                                     double *err_dev, void *stream) {
+  qh_stream(stream);
+  ensure_weid();
+  if (ntrip <= 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  const dim3 g(ntrip), b(64);
+#define QH_SS(L) case L: if (xy) hipLaunchKernelGGL((k_sigma_short<L, true>), g, b, 0, s, ntrip, tr_dev, npts, pts_dev, eta, \
+      leg0_dev, csphase, (v2d *)out_dev, err_dev); else hipLaunchKernelGGL((k_sigma_short<L, false>), g, b, 0, s, ntrip, tr_dev, \
+      npts, pts_dev, eta, leg0_dev, csphase, (v2d *)out_dev, err_dev); break;
+  switch (LE) {
+    QH_SS(3) QH_SS(5) QH_SS(7) QH_SS(9)
+    default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
+  }
+#undef QH_SS
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev,
+                                    double eta, double area, const double *leg0_dev, const double *s1_dev,
+                                    const int *s1off_dev, double *out_dev, double *err_dev, void *stream) {
+  if (ntrip <= 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  const dim3 g(ntrip), b(64);
+#define QH_SL(L) case L: hipLaunchKernelGGL(k_sigma_long<L>, g, b, 0, s, ntrip, tr_dev, npts, K_dev, eta, area, leg0_dev, \
+      (const v2d *)s1_dev, s1off_dev, (v2d *)out_dev, err_dev); break;
+  switch (LE) {
+    QH_SL(3) QH_SL(5) QH_SL(7) QH_SL(9)
+    default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
+  }
+#undef QH_SL
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_sigma0(int n, const double *k_dev, double eta, double leg00, double *out_dev, void *stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_sigma0, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)qh_stream(stream), n, k_dev, eta, leg00,
+                     (v2d *)out_dev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_sigma_total(int ntrip, int nsc, const double *ss, const double *sl, const double *s0,
+                                     const int *flag0, int parts, double *tot, void *stream) {
+  if (ntrip <= 0) return 0;
+  hipLaunchKernelGGL(k_sigma_total, dim3((nsc + 63) / 64, ntrip), dim3(64), 0, (hipStream_t)qh_stream(stream), ntrip, nsc,
+                     (const v2d *)ss, (const v2d *)sl, (const v2d *)s0, flag0, parts, (v2d *)tot);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_e32_combine(const qh_trans_tables *t, int ntrip, int nsc, const double *sig_dev, int bq_mode,
+                                     double *A_dev, double *B_dev, void *stream) {
+  if (ntrip <= 0) return 0;
+  const int ne = t->nelem;
+  hipLaunchKernelGGL(k_e32_combine, dim3((ne * ne + 255) / 256, ntrip), dim3(256), 0, (hipStream_t)qh_stream(stream),
+                     ntrip, ne, t->qslots, nsc, (const v2d *)sig_dev, (const v2d *)t->dA, (const v2d *)t->dB, t->dqA,
+                     t->dqB, bq_mode, (v2d *)A_dev, (v2d *)B_dev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
