@@ -101,6 +101,8 @@ typedef struct cart2_t { double x, y; } cart2_t;                /* qpms_types.h:
 typedef struct sph_t { double r, theta, phi; } sph_t;           /* qpms_types.h:235 */
 typedef struct csph_t { _Complex double r; double theta, phi; } csph_t; /* qpms_types.h:240 */
 typedef struct pol_t { double r, phi; } pol_t;                  /* qpms_types.h:251 */
+typedef struct ccart3_t { _Complex double x, y, z; } ccart3_t;  /* qpms_types.h:219 */
+typedef struct csphvec_t { _Complex double rc, thetac, phic; } csphvec_t; /* qpms_types.h:218 */
 typedef union anycoord_point_t {                               /* qpms_types.h:256 */
 	double z; cart3_t cart3; cart2_t cart2; sph_t sph; pol_t pol;
 } anycoord_point_t;
@@ -501,6 +503,33 @@ qpms_ss_LU qpms_scatsyswk_build_modeproblem_matrix_full_LU(_Complex double *targ
 		const qpms_scatsys_at_omega_k_t *sswk);
 /* scatsystem.c:2265-2273 */
 _Complex double *qpms_scatsys_scatter_solve(_Complex double *f, const _Complex double *a_inc, qpms_ss_LU lu);
+
+/* ------------------------------------------------------------------------ */
+/* Incident fields — vswf.h:17-124, scatsystem.h:618-636                     */
+/* ------------------------------------------------------------------------ */
+/* vswf.h:17-24 */
+typedef qpms_errno_t (*qpms_incfield_t)(_Complex double *target, const qpms_vswf_set_spec_t *bspec,
+		const cart3_t evalpoint, const void *args, bool add);
+
+/* vswf.h:93-103 */
+typedef struct qpms_incfield_planewave_params_t {
+	bool use_cartesian;
+	union { ccart3_t cart; csph_t sph; } k;
+	union { ccart3_t cart; csphvec_t sph; } E;
+} qpms_incfield_planewave_params_t;
+
+/* vswf.c:480-530: regular-VSWF coefficients of a plane wave at evalpoint,
+ * computed on the GPU (default normalisation; real wave vector, as the
+ * reference requires). */
+qpms_errno_t qpms_incfield_planewave(_Complex double *target, const qpms_vswf_set_spec_t *bspec,
+		const cart3_t evalpoint, const void *args, bool add);
+/* scatsystem.c:1970-1982 (one launch for all particles when f is
+ * qpms_incfield_planewave and every particle has the default basis). */
+_Complex double *qpms_scatsys_incident_field_vector_full(_Complex double *target_full, const qpms_scatsys_t *ss,
+		qpms_incfield_t field_at_point, const void *args, bool add);
+/* scatsystem.c:1996-2011 */
+_Complex double *qpms_scatsysw_apply_Tmatrices_full(_Complex double *target_full, const _Complex double *inc_full,
+		const qpms_scatsys_at_omega_t *ssw);
 
 /* ------------------------------------------------------------------------ */
 /* Beyn contour eigensolver — beyn.h:11-126                                  */

@@ -86,4 +86,10 @@ double or_sr_integral(double r, double k, int n, double eta);
 void or_e32_combine(const or_calc *c, const double complex *sigma_total, int bqmax_mode,
 		double complex *A, double complex *B);
 
+/* Plane-wave regular-VSWF coefficients at npart points (vswf.c:445-530):
+ * out[p*2*nelem + y] electric, out[p*2*nelem + nelem + y] magnetic; real
+ * wave vector kvec, complex cartesian amplitude E[3]. */
+void or_planewave(int lMax, int npart, const double *pos, const double *kvec, const double complex *E,
+		double complex *out);
+
 #endif

@@ -226,3 +226,86 @@ def solve_mixed(A, B, tol=1e-14, maxit=30):
     dX = DeviceBuffer(B.nbytes)
     it = lib().qpms_hip_solve_mixed_dev(n, nrhs, dA.ptr, dB.ptr, dX.ptr, float(tol), int(maxit), None)
     return dX.to_array(np.complex128, B.shape), it
+
+
+# ---------------------------------------------------------------------------
+# Beyn contour eigensolver (beyn.h:11-126)
+# ---------------------------------------------------------------------------
+class beyn_result_t(ctypes.Structure):
+    """beyn.h:82-94"""
+    _fields_ = [("neig", ctypes.c_size_t), ("vlen", ctypes.c_size_t),
+                ("eigval", ctypes.POINTER(c_cplx)), ("eigval_err", ctypes.POINTER(c_cplx)),
+                ("residuals", ctypes.POINTER(ctypes.c_double)), ("eigvec", ctypes.POINTER(c_cplx)),
+                ("ranktest_SV", ctypes.POINTER(ctypes.c_double)), ("gsl", ctypes.c_void_p)]
+
+
+# int (*)(complex double *target_M, size_t m, complex double z, void *params), beyn.h:14-16
+beyn_function_M_t = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, c_cplx, ctypes.c_void_p)
+
+
+def _beyn_bind():
+    L = lib()
+    if getattr(L, "_beyn_bound", False):
+        return L
+    vp, sz, dbl = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double
+    L.beyn_contour_ellipse.restype = vp
+    L.beyn_contour_ellipse.argtypes = [c_cplx, dbl, dbl, sz]
+    L.beyn_solve_v.restype = ctypes.POINTER(beyn_result_t)
+    L.beyn_solve_v.argtypes = [sz, sz, beyn_function_M_t, vp, vp, vp, dbl, sz, dbl, vp]
+    L.beyn_solve.restype = ctypes.POINTER(beyn_result_t)
+    L.beyn_solve.argtypes = [sz, sz, beyn_function_M_t, vp, vp, vp, dbl, sz, dbl]
+    L.beyn_result_free.argtypes = [ctypes.POINTER(beyn_result_t)]
+    L._beyn_bound = True
+    return L
+
+
+def _libc_free(p):
+    ctypes.CDLL(None).free(ctypes.c_void_p(p))
+
+
+class BeynResult:
+    """Host copy of a beyn_result_t: eigval, eigval_err, residuals, eigvec
+    (neig x vlen) and ranktest_SV (l singular values of the fine A0)."""
+
+    def __init__(self, r, l):
+        c = r.contents
+        n, m = c.neig, c.vlen
+        self.neig, self.vlen = n, m
+
+        def carr(p, k):
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), (2 * k,)).copy().view(
+                np.complex128) if k else np.zeros(0, np.complex128)
+        self.eigval = carr(c.eigval, n)
+        self.eigval_err = carr(c.eigval_err, n)
+        self.residuals = np.ctypeslib.as_array(c.residuals, (n,)).copy() if n else np.zeros(0)
+        self.eigvec = carr(c.eigvec, n * m).reshape(n, m)
+        self.ranktest_SV = np.ctypeslib.as_array(c.ranktest_SV, (l,)).copy() if l else np.zeros(0)
+
+
+def beyn_solve(Mfun, m, l, centre, halfax_re, halfax_im, npoints, rank_tol=1e-4, rank_sel_min=1, res_tol=0.0,
+               Vhat=None):
+    """Run beyn_solve (seeded probe) or beyn_solve_v (given probe Vhat, m x l)
+    with M(z) supplied by the Python callable Mfun(z) -> (m, m) complex array."""
+    L = _beyn_bind()
+
+    def cb(target, mm, z, params):
+        A = np.ascontiguousarray(Mfun(complex(z.re, z.im)), dtype=np.complex128)
+        if A.shape != (mm, mm):
+            return 1
+        ctypes.memmove(target, A.ctypes.data, A.nbytes)
+        return 0
+    fn = beyn_function_M_t(cb)
+    cont = L.beyn_contour_ellipse(c_cplx.of(centre), float(halfax_re), float(halfax_im), int(npoints))
+    try:
+        if Vhat is None:
+            r = L.beyn_solve(m, l, fn, None, None, cont, float(rank_tol), int(rank_sel_min), float(res_tol))
+        else:
+            V = np.ascontiguousarray(Vhat, dtype=np.complex128)
+            assert V.shape == (m, l)
+            r = L.beyn_solve_v(m, l, fn, None, None, cont, float(rank_tol), int(rank_sel_min), float(res_tol),
+                               V.ctypes.data)
+        out = BeynResult(r, l)
+        L.beyn_result_free(r)
+    finally:
+        _libc_free(cont)
+    return out

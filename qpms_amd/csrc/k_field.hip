@@ -1,0 +1,177 @@
+// Incident-field and T-application kernels of the scattering solve.
+//
+//  k_planewave   regular-VSWF expansion of a plane wave at each particle
+//                position (vswf.c:445-530, legendre.c:54-110, normalisation.h:140-210),
+//                default normalisation (power-normalised, Condon–Shortley phase).
+//                One thread per (particle, y); output in the default basis
+//                order: electric block (y = 0..nelem-1), then magnetic block.
+//  k_apply_t     f_p = T_{type(p)} a_p for dense nb x nb T per particle type
+//                (scatsystem.c:1996-2011 with qpms_apply_tmatrix), one thread
+//                per output row.
+//  k_cdot        -Re(a^H f) partial sums for the extinction cross-section
+//                (misc/finiterectlat-scatter.py:108), one block per batch item.
+#include <hip/hip_runtime.h>
+#include "dev_specfunc.hpp"
+#include "qpms_internal.h"
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+namespace {
+
+// Ferrers P_l^m (CS phase included) and its x-derivative for l = 1..LMAX,
+// m = 0..l, scaled to the spherical-harmonic normalisation divided by
+// sqrt(l(l+1)) as qpms_pitau_fill does (legendre.c:89-99).
+template <int LMAX>
+__device__ void pitau_lm(int l, int m, double x, double st, double &leg, double &pi, double &tau) {
+  constexpr double PI_ = 3.14159265358979323846;
+  const int am = m < 0 ? -m : m;
+  if (fabs(x) == 1.0) {
+    // DLMF 14.8.2 limits (legendre.c:65-80)
+    leg = 0; pi = 0; tau = 0;
+    if (m == 0) leg = ((l & 1) ? x : 1.0) * sqrt((2 * l + 1) / (4 * PI_ * l * (l + 1)));
+    if (am == 1) {
+      const double fl = 0.25 * sqrt((2 * l + 1) / PI_);
+      const double lpar = (l & 1) ? -1.0 : 1.0;
+      const double cs = -1.0;
+      pi = -((x > 0) ? -1.0 : lpar) * fl * cs;
+      tau = ((m > 0) ? 1.0 : -1.0) * ((x > 0) ? 1.0 : lpar) * fl * cs;
+    }
+    return;
+  }
+  double P[(LMAX + 1) * (LMAX + 2) / 2];
+  qsf::legendre_cs<LMAX>(x, st, P);
+  const double Plm = P[qsf::leg_idx(l, am)];
+  const double Pl1 = (l - 1 >= am) ? P[qsf::leg_idx(l - 1, am)] : 0.0;
+  // (1 - x^2) dP_l^m/dx = (l + m) P_{l-1}^m - l x P_l^m (DLMF 14.10.5)
+  const double dP = ((l + am) * Pl1 - l * x * Plm) / ((1.0 - x) * (1.0 + x));
+  // sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) / sqrt(l(l+1))
+  double fr = 1.0;
+  for (int t = l - am + 1; t <= l + am; ++t) fr *= t;
+  double N = sqrt((2 * l + 1) / (4 * PI_ * fr)) / sqrt((double)(l * (l + 1)));
+  if (m < 0 && (am & 1)) N = -N;  // negative m: (-1)^m (legendre.c:25-33)
+  leg = N * Plm;
+  const double legder = N * dP;
+  pi = m / st * leg;
+  tau = -st * legder;
+}
+
+template <int LMAX>
+__global__ void k_planewave(int npart, const double *__restrict__ pos, double kx, double ky, double kz,
+                            const double *__restrict__ Ecart, v2d *__restrict__ out, long ld) {
+  constexpr int NE = LMAX * (LMAX + 2);
+  constexpr double PI_ = 3.14159265358979323846;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npart * NE) return;
+  const int p = t / NE, y = t % NE;
+  int l = 1;
+  while ((l + 1) * (l + 2) - 1 <= y) ++l;
+  const int m = y - l * (l + 1) + 1;
+  // direction of Re k (cart2sph, vectors.h:134-140)
+  const double kr = sqrt(kx * kx + ky * ky + kz * kz);
+  const double theta = kr != 0.0 ? acos(kz / kr) : PI_ / 2;
+  const double phi = atan2(ky, kx);
+  const double ct = cos(theta), st = sin(theta), cf = cos(phi), sf = sin(phi);
+  double leg, pi, tau;
+  pitau_lm<LMAX>(l, m, ct, st, leg, pi, tau);
+  // E in the local geographic basis (ccart2csphvec, vectors.h:306-327)
+  const cplx Ex = mk(Ecart[0], Ecart[1]), Ey = mk(Ecart[2], Ecart[3]), Ez = mk(Ecart[4], Ecart[5]);
+  const cplx Et = (ct * cf) * Ex + (ct * sf) * Ey + (-st) * Ez;
+  const cplx Ef = (-sf) * Ex + cf * Ey;
+  // dual waves: e^{-i m phi}, derivative/m = -i e^{-i m phi} (normalisation.h:140-205)
+  const cplx eimf = cexpi(-m * phi);
+  const cplx deimf = (m == 0) ? mk(0.0, 0.0) : mk(eimf.im, -eimf.re);
+  // A1 = (0, pi d, -tau e), A2 = (0, tau e, pi d) (vswf.c:347-381)
+  const cplx dot1 = (pi * deimf) * Et + (-tau * eimf) * Ef;
+  const cplx dot2 = (tau * eimf) * Et + (pi * deimf) * Ef;
+  // 4 pi i^l and -4 pi i^(l+1) (vswf.c:455-462)
+  const cplx il = ((l & 3) == 0) ? mk(1, 0) : ((l & 3) == 1) ? mk(0, 1) : ((l & 3) == 2) ? mk(-1, 0) : mk(0, -1);
+  const cplx il1 = mk(-il.im, il.re);
+  const cplx mg = (4 * PI_) * il * dot1;
+  const cplx el = (-4 * PI_) * il1 * dot2;
+  // phase e^{i k.r} of the expansion point (vswf.c:486-488)
+  const double ph = kx * pos[3 * p] + ky * pos[3 * p + 1] + kz * pos[3 * p + 2];
+  const cplx f = cexpi(ph);
+  const cplx e1 = el * f, m1 = mg * f;
+  out[(size_t)p * ld + y] = v2d{e1.re, e1.im};
+  out[(size_t)p * ld + NE + y] = v2d{m1.re, m1.im};
+}
+
+// out[p] = T[ttype[p]] a[p], T row-major nb x nb per type.
+__global__ void k_apply_t(int npart, int nb, const int *__restrict__ ttype, const v2d *__restrict__ T,
+                          const v2d *__restrict__ a, v2d *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npart * nb) return;
+  const int p = t / nb, r = t % nb;
+  const v2d *Tr = T + ((size_t)ttype[p] * nb + r) * nb;
+  const v2d *ap = a + (size_t)p * nb;
+  double sr = 0, si = 0;
+  for (int c = 0; c < nb; ++c) {
+    const v2d x = Tr[c], v = ap[c];
+    sr += x.x * v.x - x.y * v.y;
+    si += x.x * v.y + x.y * v.x;
+  }
+  out[(size_t)p * nb + r] = v2d{sr, si};
+}
+
+// partial[b] = -Re(sum_i conj(a_i) f_i) over n entries of batch item b.
+__global__ void k_neg_re_cdot(long n, const v2d *__restrict__ a, long sa, const v2d *__restrict__ f, long sf,
+                              double *__restrict__ out) {
+  __shared__ double red[256];
+  const v2d *ab = a + (size_t)blockIdx.x * sa, *fb = f + (size_t)blockIdx.x * sf;
+  double s = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const v2d x = ab[i], y = fb[i];
+    s += x.x * y.x + x.y * y.y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = -red[0];
+}
+
+}  // namespace
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+  qpms_hip_die(__FILE__, __LINE__, __func__, "HIP error %d (%s) in %s", (int)e_, hipGetErrorString(e_), #x); } while (0)
+
+extern "C" int qh_launch_planewave(int lMax, size_t npart, const double *pos_dev, const double kvec[3],
+                                   const double *E_dev, double *out_dev, long ld, void *stream) {
+  if (npart == 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  const int ne = lMax * (lMax + 2);
+  const dim3 grid((unsigned)((npart * ne + 255) / 256)), block(256);
+  v2d *o = (v2d *)out_dev;
+  switch (lMax) {
+    case 1: hipLaunchKernelGGL(k_planewave<1>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 2: hipLaunchKernelGGL(k_planewave<2>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 3: hipLaunchKernelGGL(k_planewave<3>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 4: hipLaunchKernelGGL(k_planewave<4>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", lMax);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_apply_t(size_t npart, int nb, const int *ttype_dev, const double *T_dev, const double *a_dev,
+                                 double *out_dev, void *stream) {
+  if (npart == 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  const dim3 grid((unsigned)((npart * nb + 255) / 256)), block(256);
+  hipLaunchKernelGGL(k_apply_t, grid, block, 0, s, (int)npart, nb, ttype_dev, (const v2d *)T_dev,
+                     (const v2d *)a_dev, (v2d *)out_dev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_launch_neg_re_cdot(int batch, long n, const double *a_dev, long sa, const double *f_dev, long sf,
+                                     double *out_dev, void *stream) {
+  if (batch == 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  hipLaunchKernelGGL(k_neg_re_cdot, dim3(batch), dim3(256), 0, s, n, (const v2d *)a_dev, sa, (const v2d *)f_dev, sf,
+                     out_dev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}

@@ -556,3 +556,142 @@ complex double *qpms_scatsys_scatter_solve(complex double *f, const complex doub
 	qh_dfree(dA); qh_dfree(dp);
 	return f;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Incident field and T application (vswf.c:480-530, scatsystem.c:1970-2011) */
+/* ---------------------------------------------------------------------- */
+int qh_launch_planewave(int lMax, size_t npart, const double *pos_dev, const double kvec[3],
+		const double *E_dev, double *out_dev, long ld, void *stream);
+int qh_launch_apply_t(size_t npart, int nb, const int *ttype_dev, const double *T_dev, const double *a_dev,
+		double *out_dev, void *stream);
+
+/* Real k and cartesian E of the parameter structure (vswf.c:483-496). */
+static void qh_planewave_kE(const qpms_incfield_planewave_params_t *p, double kv[3], double E[6]) {
+	complex double kx, ky, kz, ex, ey, ez;
+	if (p->use_cartesian) {
+		kx = p->k.cart.x; ky = p->k.cart.y; kz = p->k.cart.z;
+		ex = p->E.cart.x; ey = p->E.cart.y; ez = p->E.cart.z;
+	} else {
+		const double st = sin(p->k.sph.theta), ct = cos(p->k.sph.theta);
+		const double sf = sin(p->k.sph.phi), cf = cos(p->k.sph.phi);
+		const complex double r = p->k.sph.r;
+		kx = r * st * cf; ky = r * st * sf; kz = r * ct;
+		/* csphvec2ccart at the wave direction (vectors.h:283-298) */
+		const complex double er = p->E.sph.rc, et = p->E.sph.thetac, ef = p->E.sph.phic;
+		ex = st * cf * er + ct * cf * et - sf * ef;
+		ey = st * sf * er + ct * sf * et + cf * ef;
+		ez = ct * er - st * et;
+	}
+	QH_ENSURE(cimag(kx) == 0 && cimag(ky) == 0 && cimag(kz) == 0,
+		"Complex-valued wave vector not implemented correctly; cf. docs.");
+	kv[0] = creal(kx); kv[1] = creal(ky); kv[2] = creal(kz);
+	E[0] = creal(ex); E[1] = cimag(ex); E[2] = creal(ey); E[3] = cimag(ey); E[4] = creal(ez); E[5] = cimag(ez);
+}
+
+/* Default-basis coefficients (electric block, magnetic block) at np points. */
+static void qh_planewave_points(int lMax, size_t np, const double *pos_host, const double kv[3], const double E[6],
+		complex double *out) {
+	const size_t nb = 2 * (size_t)qpms_lMax2nelem(lMax);
+	double *dpos = qh_dmalloc(3 * np * sizeof(double) + 6 * sizeof(double));
+	double *dE = dpos + 3 * np;
+	complex double *dout = qh_dmalloc(np * nb * sizeof(complex double));
+	qh_h2d(dpos, pos_host, 3 * np * sizeof(double), NULL);
+	qh_h2d(dE, E, 6 * sizeof(double), NULL);
+	qh_launch_planewave(lMax, np, dpos, kv, dE, (double *)dout, (long)nb, NULL);
+	qh_d2h(out, dout, np * nb * sizeof(complex double), NULL);
+	qh_sync(NULL);
+	qh_dfree(dout); qh_dfree(dpos);
+}
+
+/* Scatter default-basis coefficients into bspec order (vswf.c:504-526). */
+static void qh_planewave_to_bspec(complex double *target, const qpms_vswf_set_spec_t *bspec, const complex double *c,
+		size_t nelem, bool add) {
+	if (!add) memset(target, 0, bspec->n * sizeof(complex double));
+	for (size_t i = 0; i < bspec->n; ++i) {
+		const qpms_uvswfi_t ui = bspec->ilist[i];
+		const int t = (int)(ui % 4);
+		const size_t ysc = (size_t)(ui / 4);
+		if (ysc == 0) { target[i] = 0; continue; }
+		const size_t y = ysc - 1;
+		QH_ENSURE(t != QPMS_VSWF_LONGITUDINAL, "longitudinal incident-field components are not supported");
+		target[i] += (t == QPMS_VSWF_ELECTRIC) ? c[y] : c[nelem + y];
+	}
+}
+
+qpms_errno_t qpms_incfield_planewave(complex double *target, const qpms_vswf_set_spec_t *bspec,
+		const cart3_t evalpoint, const void *args, bool add) {
+	QH_ENSURE(bspec->norm == QPMS_NORMALISATION_DEFAULT, "only the default VSWF normalisation is supported");
+	double kv[3], E[6];
+	qh_planewave_kE((const qpms_incfield_planewave_params_t *)args, kv, E);
+	const size_t ne = qpms_lMax2nelem(bspec->lMax);
+	complex double *c;
+	QH_MALLOC(c, 2 * ne * sizeof(complex double));
+	const double pos[3] = {evalpoint.x, evalpoint.y, evalpoint.z};
+	qh_planewave_points(bspec->lMax, 1, pos, kv, E, c);
+	qh_planewave_to_bspec(target, bspec, c, ne, add);
+	free(c);
+	return QPMS_SUCCESS;
+}
+
+complex double *qpms_scatsys_incident_field_vector_full(complex double *target_full, const qpms_scatsys_t *ss,
+		qpms_incfield_t f, const void *args, bool add) {
+	if (!target_full) QH_CALLOC(target_full, ss->fecv_size, sizeof(complex double));
+	if (f == qpms_incfield_planewave && qh_uniform_default_bspec(ss) && ss->c->normalisation == QPMS_NORMALISATION_DEFAULT) {
+		/* every particle at once on the GPU */
+		double kv[3], E[6];
+		qh_planewave_kE((const qpms_incfield_planewave_params_t *)args, kv, E);
+		const size_t np = ss->p_count, nb = 2 * ss->c->nelem;
+		double *pos;
+		QH_MALLOC(pos, 3 * np * sizeof(double));
+		for (size_t i = 0; i < np; ++i) {
+			pos[3 * i] = ss->p[i].pos.x; pos[3 * i + 1] = ss->p[i].pos.y; pos[3 * i + 2] = ss->p[i].pos.z;
+		}
+		complex double *c;
+		QH_MALLOC(c, np * nb * sizeof(complex double));
+		qh_planewave_points(ss->c->lMax, np, pos, kv, E, c);
+		for (size_t i = 0; i < np * nb; ++i) target_full[i] = add ? target_full[i] + c[i] : c[i];
+		free(c); free(pos);
+		return target_full;
+	}
+	for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi)
+		QH_ENSURE(f(target_full + ss->fecv_pstarts[pi], qh_bspec_pi(ss, pi), ss->p[pi].pos, args, add) == QPMS_SUCCESS,
+			"incident field evaluation failed");
+	return target_full;
+}
+
+complex double *qpms_scatsysw_apply_Tmatrices_full(complex double *target_full, const complex double *inc_full,
+		const qpms_scatsys_at_omega_t *ssw) {
+	const qpms_scatsys_t *ss = ssw->ss;
+	if (!target_full) QH_CALLOC(target_full, ss->fecv_size, sizeof(complex double));
+	/* group particles by T (tm index); every particle's block is nb x nb */
+	for (qpms_ss_tmi_t ti = 0; ti < ss->tm_count; ++ti) {
+		const qpms_tmatrix_t *T = ssw->tm[ti];
+		const size_t nb = T->spec->n;
+		size_t cnt = 0;
+		for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi) cnt += (ss->p[pi].tmatrix_id == ti);
+		if (!cnt) continue;
+		complex double *a, *o;
+		int *ty;
+		QH_MALLOC(a, cnt * nb * sizeof(complex double));
+		QH_MALLOC(o, cnt * nb * sizeof(complex double));
+		QH_CALLOC(ty, cnt, sizeof(int));
+		size_t j = 0;
+		for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi)
+			if (ss->p[pi].tmatrix_id == ti) memcpy(a + nb * j++, inc_full + ss->fecv_pstarts[pi], nb * sizeof(complex double));
+		complex double *dT = qh_dmalloc((nb * nb + 2 * cnt * nb) * sizeof(complex double));
+		complex double *da = dT + nb * nb, *dout = da + cnt * nb;
+		int *dty = qh_dmalloc(cnt * sizeof(int));
+		qh_h2d(dT, T->m, nb * nb * sizeof(complex double), NULL);
+		qh_h2d(da, a, cnt * nb * sizeof(complex double), NULL);
+		qh_h2d(dty, ty, cnt * sizeof(int), NULL);
+		qh_launch_apply_t(cnt, (int)nb, dty, (const double *)dT, (const double *)da, (double *)dout, NULL);
+		qh_d2h(o, dout, cnt * nb * sizeof(complex double), NULL);
+		qh_sync(NULL);
+		j = 0;
+		for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi)
+			if (ss->p[pi].tmatrix_id == ti) memcpy(target_full + ss->fecv_pstarts[pi], o + nb * j++, nb * sizeof(complex double));
+		qh_dfree(dT); qh_dfree(dty);
+		free(a); free(o); free(ty);
+	}
+	return target_full;
+}

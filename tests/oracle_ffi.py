@@ -148,3 +148,50 @@ def zgetrs(LU, ipiv, B):
     oracle().or_zgetrs(n, nrhs, np.ascontiguousarray(LU).ctypes.data, np.ascontiguousarray(ipiv, np.int32).ctypes.data,
                        B.ctypes.data)
     return B
+
+
+EV = 1.602176634e-19 / 1.054571817e-34   # rad/s per eV
+C0 = 2.99792458e8
+
+
+def planewave(lMax, pos, kvec, E):
+    """Plane-wave coefficients (npart, 2*nelem): electric block then magnetic."""
+    o = oracle()
+    o.or_planewave.argtypes = [i32, i32, vp, vp, vp, vp]
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    kvec = np.ascontiguousarray(kvec, np.float64)
+    E = np.ascontiguousarray(E, np.complex128)
+    ne = lMax * (lMax + 2)
+    out = np.empty((len(pos), 2 * ne), np.complex128)
+    o.or_planewave(lMax, len(pos), pos.ctypes.data, kvec.ctypes.data, E.ctypes.data, out.ctypes.data)
+    return out
+
+
+def mie_tdiag(lMax, radius, eps_in, eps_bg, omega):
+    """Diagonal of the sphere T-matrix in the default basis (tmatrices.c:500-591)."""
+    ke = np.sqrt(complex(eps_bg)) * omega / C0
+    ki = np.sqrt(complex(eps_in)) * omega / C0
+    RV, RH = mie(lMax, radius, ki, ke)
+    el = np.concatenate([[RV[l]] * (2 * l + 1) for l in range(1, lMax + 1)])
+    mg = np.concatenate([[RH[l]] * (2 * l + 1) for l in range(1, lMax + 1)])
+    return np.concatenate([el, mg]), ke
+
+
+def finite_cluster(pos, lMax, radius, eps_in, eps_bg, eV, kdir, E):
+    """Oracle pipeline of a plane-wave scattering solve on a finite cluster of
+    identical spheres: W, M = I - T W, a, T a, f = M^-1 T a, and the
+    extinction cross-section -Re(a^H f)/k^2 (misc/finiterectlat-scatter.py:104-109)."""
+    omega = eV * EV
+    tdiag, k = mie_tdiag(lMax, radius, eps_in, eps_bg, omega)
+    oc = OracleCalc(lMax)
+    pos = np.ascontiguousarray(pos, np.float64)
+    n = len(pos)
+    W = oc.build_W_rows(pos, k, 0, n)
+    M = oc.build_M(pos, np.zeros(n, np.int32), tdiag, k)
+    a = planewave(lMax, pos, k.real * np.asarray(kdir, float), E).ravel()
+    Ta = np.tile(tdiag, n) * a
+    LU, ipiv, info = zgetrf(M)
+    assert info == 0
+    f = zgetrs(LU, ipiv, Ta)
+    sigma = -np.vdot(a, f).real / (k * k.conjugate()).real
+    return dict(W=W, M=M, a=a, Ta=Ta, f=f, tdiag=tdiag, k=k, sigma_ext=sigma)

@@ -1,0 +1,111 @@
+"""GPU parity of the plane-wave scattering solve on a finite cluster, through
+the public C ABI (qpms_incfield_planewave, qpms_scatsys_incident_field_vector_full,
+qpms_scatsysw_apply_Tmatrices_full, the W / M builders, LU and
+qpms_scatsys_scatter_solve), against the oracle pipeline.
+
+The 4x4 cluster is the plain finite-cluster configuration: pitch 500 nm,
+spheres r = 50 nm, eps = -12+1i in eps = 2.31, lMax = 2 (L = 256), 2.0 eV,
+x-polarised plane wave at normal incidence. Tolerances: 1e-10 relative for
+builds and incident coefficients, 1e-9 for the solve and the extinction."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "c", "_build", "ss_finite")
+
+
+class ccart3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double * 2), ("y", ctypes.c_double * 2), ("z", ctypes.c_double * 2)]
+
+
+class pw_params(ctypes.Structure):
+    """vswf.h:93-103 (both unions' largest member is ccart3_t)."""
+    _fields_ = [("use_cartesian", ctypes.c_bool), ("k", ccart3), ("E", ccart3)]
+
+
+def _c3(v):
+    v = np.asarray(v, complex)
+    return ccart3((ctypes.c_double * 2)(v[0].real, v[0].imag), (ctypes.c_double * 2)(v[1].real, v[1].imag),
+                  (ctypes.c_double * 2)(v[2].real, v[2].imag))
+
+
+DIRS = [((0, 0, 1.0), (1, 0, 0)), ((0, 0, -1.0), (0, 1j, 0)), ((0.3, 0.4, np.sqrt(0.75)), (0.8, -0.6, 0)),
+        ((1.0, 0, 0), (0, 0.6, 0.8j))]
+
+
+@pytest.mark.parametrize("lMax", [1, 2, 3, 4])
+@pytest.mark.parametrize("case", range(len(DIRS)))
+def test_incfield_planewave(gpu, lMax, case):
+    import qpms_amd
+    L = qpms_amd.lib()
+    L.qpms_vswf_set_spec_from_lMax.restype = ctypes.c_void_p
+    L.qpms_vswf_set_spec_from_lMax.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.qpms_vswf_set_spec_free.argtypes = [ctypes.c_void_p]
+    L.qpms_incfield_planewave.argtypes = [ctypes.c_void_p, ctypes.c_void_p, qpms_amd.cart3_t, ctypes.c_void_p,
+                                          ctypes.c_bool]
+    kd, E = DIRS[case]
+    k = 1.1e7 * np.asarray(kd)
+    bspec = L.qpms_vswf_set_spec_from_lMax(lMax, qpms_amd.NORM_DEFAULT)
+    ne = lMax * (lMax + 2)
+    pos = np.array([[120e-9, -40e-9, 75e-9]])
+    p = pw_params(True, _c3(k), _c3(E))
+    out = np.full(2 * ne, 1 + 1j)
+    L.qpms_incfield_planewave(out.ctypes.data, bspec, qpms_amd.cart3_t(*pos[0]), ctypes.byref(p), False)
+    ref = of.planewave(lMax, pos, k, np.asarray(E, complex))[0]
+    assert np.max(np.abs(out - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # add = true accumulates
+    L.qpms_incfield_planewave(out.ctypes.data, bspec, qpms_amd.cart3_t(*pos[0]), ctypes.byref(p), True)
+    assert np.max(np.abs(out - 2 * ref)) <= 2e-12 * np.max(np.abs(ref))
+    L.qpms_vswf_set_spec_free(bspec)
+
+
+def run_driver(tmp_path, nx, ny, pitch, radius, eps_in, eps_bg, eV, lMax, kd, E):
+    out = tmp_path / "ss.bin"
+    args = [DRIVER, str(out), str(nx), str(ny), repr(pitch), repr(radius), repr(eps_in.real), repr(eps_in.imag),
+            repr(eps_bg), repr(eV), str(lMax)] + [repr(float(x)) for x in kd]
+    for e in E:
+        args += [repr(complex(e).real), repr(complex(e).imag)]
+    subprocess.run(args, check=True, timeout=120)
+    raw = out.read_bytes()
+    L = int(np.frombuffer(raw[:8], np.int64)[0])
+    nb = L // (nx * ny)
+    c = np.frombuffer(raw[8:8 + 16 * (2 * L * L + 3 * L + nb)], np.complex128)
+    tail = np.frombuffer(raw[8 + 16 * (2 * L * L + 3 * L + nb):], np.float64)
+    W = c[:L * L].reshape(L, L)
+    M = c[L * L:2 * L * L].reshape(L, L)
+    a, Ta, f = c[2 * L * L:2 * L * L + L], c[2 * L * L + L:2 * L * L + 2 * L], c[2 * L * L + 2 * L:2 * L * L + 3 * L]
+    td = c[2 * L * L + 3 * L:]
+    return dict(W=W, M=M, a=a, Ta=Ta, f=f, tdiag=td, sigma_ext=tail[0], k=complex(tail[1], tail[2]))
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("shape,lMax,kd,E", [
+    ((4, 4), 2, (0, 0, 1.0), (1, 0, 0)),                       # plain finite-cluster configuration
+    ((3, 2), 3, (0.3, 0.4, np.sqrt(0.75)), (0.8, -0.6, 0)),    # oblique incidence, lMax 3
+    ((2, 2), 4, (0, 0, 1.0), (0, 1, 0)),
+])
+def test_finite_cluster_abi(gpu, tmp_path, shape, lMax, kd, E):
+    nx, ny = shape
+    pitch, radius, eps_in, eps_bg, eV = 500e-9, 50e-9, -12 + 1j, 2.31, 2.0
+    g = run_driver(tmp_path, nx, ny, pitch, radius, eps_in, eps_bg, eV, lMax, kd, E)
+    pos = np.array([[(i - 0.5 * (nx - 1)) * pitch, (j - 0.5 * (ny - 1)) * pitch, 0.0]
+                    for i in range(nx) for j in range(ny)])
+    o = of.finite_cluster(pos, lMax, radius, eps_in, eps_bg, eV, kd, np.asarray(E, complex))
+    assert abs(g["k"] - o["k"]) <= 1e-15 * abs(o["k"])
+    assert rel(g["tdiag"], o["tdiag"]) < 1e-12
+    assert rel(g["W"], o["W"]) < 1e-10
+    assert rel(g["M"], o["M"]) < 1e-10
+    assert rel(g["a"], o["a"]) < 1e-12
+    assert rel(g["Ta"], o["Ta"]) < 1e-12
+    assert rel(g["f"], o["f"]) < 1e-9
+    assert abs(g["sigma_ext"] - o["sigma_ext"]) < 1e-9 * abs(o["sigma_ext"])
