@@ -554,6 +554,12 @@ typedef enum {                                     /* beyn.h:49-54 */
 } beyn_contour_halfellipse_orientation;
 
 beyn_contour_t *beyn_contour_ellipse(_Complex double centre, double halfax_re, double halfax_im, size_t npoints); /* beyn.c:69-86 */
+/* beyn.c:120-163: half-ellipse; n/2 nodes on the straight cut, the rest on the arc. */
+beyn_contour_t *beyn_contour_halfellipse(_Complex double centre, double halfax_re, double halfax_im, size_t npoints,
+		beyn_contour_halfellipse_orientation orientation);
+/* beyn.c:165-248: half-ellipse with rounded corners, rounding in [0, 0.5). */
+beyn_contour_t *beyn_contour_kidney(_Complex double centre, double halfax_re, double halfax_im,
+		double rounding, size_t n, beyn_contour_halfellipse_orientation orientation);
 
 /* beyn.h:82-94; `gsl` is this library's private bookkeeping pointer. */
 typedef struct beyn_result_t {
@@ -575,6 +581,11 @@ beyn_result_t *beyn_solve(size_t m, size_t l, beyn_function_M_t M, beyn_function
 
 /* scatsystem.c:2301-2325 (full system only: iri must be QPMS_NO_IRREP). */
 struct beyn_result_t *qpms_scatsys_finite_find_eigenmodes(const qpms_scatsys_t *ss, qpms_iri_t iri,
+		_Complex double omega_centre, double omega_rr, double omega_ri, size_t contour_npoints,
+		double rank_tol, size_t rank_sel_min, double res_tol);
+/* scatsystem.c:2351-2370. Like the finite wrapper, uses l = min(m, 64) probe
+ * columns (QPMS_HIP_BEYN_L) instead of l = m. */
+struct beyn_result_t *qpms_scatsys_periodic_find_eigenmodes(const qpms_scatsys_t *ss, const double k[3],
 		_Complex double omega_centre, double omega_rr, double omega_ri, size_t contour_npoints,
 		double rank_tol, size_t rank_sel_min, double res_tol);
 
@@ -636,7 +647,8 @@ int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, co
 /* Batched Ewald sums: ntrip (beta, k, s) triples on one 2D lattice in the xy
  * plane. triples_dev holds 8 doubles per triple {beta_x, beta_y, beta_z,
  * Re k, Im k, s_x, s_y, s_z}; R_dev nR direct-lattice points (x, y, z), the
- * triple's s subtracted in-kernel; K_dev nK reciprocal points (x, y).
+ * triple's s subtracted in-kernel (a point with R = s contributes nothing, so
+ * one list including the origin serves s = 0 and s != 0 alike); K_dev nK reciprocal points (x, y).
  * Outputs (either may be NULL) hold c->nelem_sc complex values per triple,
  * the same quantities as ewald3_sigma_short / ewald3_sigma_long. */
 int qpms_hip_ewald3_sigma_batch_dev(const qpms_ewald3_constants_t *c, double eta, double unitcell_area,

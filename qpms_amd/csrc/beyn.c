@@ -31,6 +31,14 @@ int qh_v0s(int m, int K, const double *S, const double *Ut, double *Y, void *str
 int qh_small_eig(int K, const double *B, double *lam, double *S, void *stream);
 int qh_bmat(int K, int l, const double *T, const double *Vt, const double *sigma, double *B, void *stream);
 
+/* (Re(z - c)/rRe)^2 + (Im(z - c)/rIm)^2 < 1 with the half-axes stored after
+ * the nodes (beyn.c:62-67). */
+static bool qh_ellipse_inside(struct beyn_contour_t *c, complex double z) {
+	const double rRe = creal(c->z_dz[c->n][0]), rIm = creal(c->z_dz[c->n][1]);
+	const double a = creal(z - c->centre) / rRe, b = cimag(z - c->centre) / rIm;
+	return a * a + b * b < 1;
+}
+
 beyn_contour_t *beyn_contour_ellipse(complex double centre, double rRe, double rIm, size_t n) {
 	beyn_contour_t *c;
 	QH_MALLOC(c, sizeof(beyn_contour_t) + (n + 1) * sizeof(c->z_dz[0]));
@@ -43,6 +51,116 @@ beyn_contour_t *beyn_contour_ellipse(complex double centre, double rRe, double r
 	}
 	c->z_dz[n][0] = rRe;   /* half-axes kept after the nodes, as the reference does */
 	c->z_dz[n][1] = rIm;
+	c->inside_test = qh_ellipse_inside;
+	return c;
+}
+
+/* Rotation taking the "upper half" (IM_PLUS) shape to the requested
+ * orientation, and which half-axis runs along the straight cut
+ * (beyn.c:127-147). */
+static complex double qh_half_orient(beyn_contour_halfellipse_orientation o, double rRe, double rIm,
+		double *l, double *h) {
+	*l = rRe; *h = rIm;
+	switch (o) {
+		case BEYN_CONTOUR_HALFELLIPSE_RE_PLUS: *l = rIm; *h = rRe; return -I;
+		case BEYN_CONTOUR_HALFELLIPSE_RE_MINUS: *l = rIm; *h = rRe; return I;
+		case BEYN_CONTOUR_HALFELLIPSE_IM_PLUS: return 1;
+		case BEYN_CONTOUR_HALFELLIPSE_IM_MINUS: return -1;
+	}
+	QH_DIE("invalid half-ellipse orientation %d", (int)o);
+	return 0;
+}
+
+/* Nodes on the straight cut get the signed zero of the side the contour
+ * encloses, so that functions with a branch cut there see the right side
+ * (beyn.c:91-117). */
+static complex double qh_align_zero(complex double z, beyn_contour_halfellipse_orientation o) {
+	double re = creal(z), im = cimag(z);
+	switch (o) {
+		case BEYN_CONTOUR_HALFELLIPSE_RE_PLUS: if (re == 0) re = +0.0; break;
+		case BEYN_CONTOUR_HALFELLIPSE_RE_MINUS: if (re == 0) re = -0.0; break;
+		case BEYN_CONTOUR_HALFELLIPSE_IM_PLUS: if (im == 0) im = +0.0; break;
+		case BEYN_CONTOUR_HALFELLIPSE_IM_MINUS: if (im == 0) im = -0.0; break;
+	}
+	return CMPLX(re, im);
+}
+
+beyn_contour_t *beyn_contour_halfellipse(complex double centre, double rRe, double rIm, size_t n,
+		beyn_contour_halfellipse_orientation o) {
+	beyn_contour_t *c;
+	QH_MALLOC(c, sizeof(beyn_contour_t) + (n + 1) * sizeof(c->z_dz[0]) + sizeof(beyn_contour_halfellipse_orientation));
+	c->centre = centre;
+	c->n = n;
+	double l, h;
+	const complex double f = qh_half_orient(o, rRe, rIm, &l, &h);
+	const size_t nline = n / 2, narc = n - nline;
+	/* arc: midpoint rule over t in (0, pi) (beyn.c:149-154) */
+	for (size_t i = 0; i < narc; ++i) {
+		const double t = (i + 0.5) * M_PI / narc;
+		c->z_dz[i][0] = centre + f * (cos(t) * l + I * sin(t) * h);
+		c->z_dz[i][1] = f * (-l * sin(t) + I * h * cos(t)) * (M_PI / narc);
+	}
+	/* cut: nline equal steps from +l back to -l (beyn.c:155-159) */
+	for (size_t i = 0; i < nline; ++i) {
+		const double t = 0.5 * (1 - (double)nline) + i;
+		c->z_dz[narc + i][0] = qh_align_zero(centre + f * t * 2 * l / nline, o);
+		c->z_dz[narc + i][1] = f * 2 * l / nline;
+	}
+	c->z_dz[n][0] = rRe;
+	c->z_dz[n][1] = rIm;
+	memcpy(&c->z_dz[n + 1][0], &o, sizeof o);
+	c->inside_test = NULL;
+	return c;
+}
+
+beyn_contour_t *beyn_contour_kidney(complex double centre, double rRe, double rIm, const double rounding,
+		const size_t n, beyn_contour_halfellipse_orientation o) {
+	QH_ENSURE(rounding >= 0 && rounding < .5, "rounding must lie in the interval [0, 0.5)");
+	beyn_contour_t *c;
+	QH_MALLOC(c, sizeof(beyn_contour_t) + (n + 1) * sizeof(c->z_dz[0]) + sizeof(beyn_contour_halfellipse_orientation));
+	c->centre = centre;
+	c->n = n;
+	double l, h;
+	const complex double f = qh_half_orient(o, rRe, rIm, &l, &h);
+	/* A half-disc of radius h (stretched by l/h along the cut) whose two
+	 * corners are replaced by arcs of radius y = rounding*h tangent to both
+	 * the cut and the big arc; the parameter is arc length in the unstretched
+	 * shape, sampled uniformly (beyn.c:199-247). */
+	const double y = rounding * h;
+	const double x = sqrt((h - y) * (h - y) - y * y);
+	const double alpha = asin(y / (h - y));
+	const double ar = l / h;
+	const double tmax = 2 * (x + (M_PI_2 + alpha) * y + (M_PI_2 - alpha) * h);
+	const double dt = tmax / n;
+	size_t i = 0;
+	double t, t0 = 0, t1 = x;
+	for (; t = i * dt, t <= t1; ++i) {                      /* cut, centre -> right corner */
+		c->z_dz[i][0] = qh_align_zero(centre + (t - t0) * ar * f, o);
+		c->z_dz[i][1] = dt * ar * f;
+	}
+	t0 = t1; t1 = t0 + (M_PI_2 + alpha) * y;
+	for (; t = i * dt, t < t1; ++i) {                       /* right rounding arc */
+		const double phi = (t - t0) / y - M_PI_2;
+		c->z_dz[i][0] = centre + (ar * (x + y * cos(phi)) + y * (1 + sin(phi)) * I) * f;
+		c->z_dz[i][1] = dt * (-ar * sin(phi) + cos(phi) * I) * f;
+	}
+	t0 = t1; t1 = t0 + (M_PI - 2 * alpha) * h;
+	for (; t = i * dt, t < t1; ++i) {                       /* big arc */
+		const double phi = (t - t0) / h + alpha;
+		c->z_dz[i][0] = centre + (ar * h * cos(phi) + h * sin(phi) * I) * f;
+		c->z_dz[i][1] = dt * (-ar * sin(phi) + cos(phi) * I) * f;
+	}
+	t0 = t1; t1 = t0 + (M_PI_2 + alpha) * y;
+	for (; t = i * dt, t < t1; ++i) {                       /* left rounding arc */
+		const double phi = (t - t0) / y + M_PI - alpha;
+		c->z_dz[i][0] = centre + (ar * (-x + y * cos(phi)) + y * (1 + sin(phi)) * I) * f;
+		c->z_dz[i][1] = dt * (-ar * sin(phi) + cos(phi) * I) * f;
+	}
+	t0 = t1;
+	for (; t = i * dt, i < n; ++i) {                        /* cut, left corner -> centre */
+		c->z_dz[i][0] = qh_align_zero(centre + (t - t0 - x) * ar * f, o);
+		c->z_dz[i][1] = dt * ar * f;
+	}
 	c->inside_test = NULL;
 	return c;
 }
@@ -69,15 +187,18 @@ static double qh_randn(uint64_t *s) {
 	return sqrt(-2 * log(u1)) * cos(2 * M_PI * u2);
 }
 
-/* Steps 3-6 of Beyn's algorithm on device A0, A1 (m x l). Returns K
- * (retained) and fills the result arrays (host). */
+/* Steps 3-6 of Beyn's algorithm on device A0, A1 (m x l), on `stream`.
+ * Returns K (retained) and fills the result arrays (host). With
+ * res_tol > 0 the residual ||M(z) v|| of every candidate is computed from the
+ * M callback and candidates above res_tol are dropped. */
 static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params,
 		const complex double *dA0, const complex double *dA1, complex double z0,
 		double rank_tol, size_t rank_sel_min, double res_tol,
-		complex double *eigval, complex double *eigvec /* l x m, nullable */, double *residuals, double *sigma) {
+		complex double *eigval, complex double *eigvec /* l x m, nullable */, double *residuals, double *sigma,
+		void *stream) {
 	complex double *dUt = qh_dmalloc(m * l * sizeof(complex double));
 	complex double *dVt = qh_dmalloc(l * l * sizeof(complex double));
-	qh_jacobi_svd((int)m, (int)l, (const double *)dA0, (double *)dUt, sigma, (double *)dVt, NULL);
+	qh_jacobi_svd((int)m, (int)l, (const double *)dA0, (double *)dUt, sigma, (double *)dVt, stream);
 	size_t K = 0;
 	for (size_t k = 0; k < l; ++k)
 		if (k < rank_sel_min || sigma[k] > rank_tol) ++K;
@@ -85,24 +206,26 @@ static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params
 	QH_ENSURE(K <= 64, "Beyn: %zu candidate eigenvalues exceed the reduced-problem limit of 64", K);
 	/* TM2 = V0^H A1 (K x l) */
 	complex double *dT = qh_dmalloc(K * l * sizeof(complex double));
-	qh_ctrans_gemm((int)m, (int)l, (int)K, (const double *)dUt, (const double *)dA1, (double *)dT, NULL);
+	qh_ctrans_gemm((int)m, (int)l, (int)K, (const double *)dUt, (const double *)dA1, (double *)dT, stream);
 	/* B = TM2 W0 Sigma^{-1}, W0 column j = row j of Vt */
 	double *dsig = qh_dmalloc(l * sizeof(double));
-	qh_h2d(dsig, sigma, l * sizeof(double), NULL);
+	qh_h2d(dsig, sigma, l * sizeof(double), stream);
 	complex double *dB = qh_dmalloc((K * K + K + K * K) * sizeof(complex double));
 	complex double *dlam = dB + K * K, *dS = dlam + K;
-	qh_bmat((int)K, (int)l, (const double *)dT, (const double *)dVt, dsig, (double *)dB, NULL);
-	int fail = qh_small_eig((int)K, (const double *)dB, (double *)dlam, (double *)dS, NULL);
+	qh_bmat((int)K, (int)l, (const double *)dT, (const double *)dVt, dsig, (double *)dB, stream);
+	int fail = qh_small_eig((int)K, (const double *)dB, (double *)dlam, (double *)dS, stream);
 	QH_ENSURE(!fail, "Beyn: the reduced eigenproblem did not converge");
 	complex double *lam;
 	QH_MALLOC(lam, K * sizeof(complex double));
-	qh_d2h(lam, dlam, K * sizeof(complex double), NULL);
-	complex double *dY = qh_dmalloc(K * m * sizeof(complex double));
-	qh_v0s((int)m, (int)K, (const double *)dS, (const double *)dUt, (double *)dY, NULL);
-	complex double *Y;
-	QH_MALLOC(Y, K * m * sizeof(complex double));
-	qh_d2h(Y, dY, K * m * sizeof(complex double), NULL);
-	qh_sync(NULL);
+	qh_d2h(lam, dlam, K * sizeof(complex double), stream);
+	complex double *dY = NULL, *Y = NULL;
+	if (eigvec || res_tol > 0) {
+		dY = qh_dmalloc(K * m * sizeof(complex double));
+		qh_v0s((int)m, (int)K, (const double *)dS, (const double *)dUt, (double *)dY, stream);
+		QH_MALLOC(Y, K * m * sizeof(complex double));
+		qh_d2h(Y, dY, K * m * sizeof(complex double), stream);
+	}
+	qh_sync(stream);
 	size_t kept = 0;
 	complex double *Mm = NULL, *dMm = NULL, *dr = NULL;
 	if (res_tol > 0) {
@@ -116,11 +239,11 @@ static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params
 		if (res_tol > 0) {
 			/* ||M(z) v|| on the GPU: r = 0 - M v */
 			QH_ENSURE(M(Mm, m, z, params) == 0, "Beyn: M callback failed");
-			qh_h2d(dMm, Mm, m * m * sizeof(complex double), NULL);
-			qh_memset(dr, 0, m * sizeof(complex double), NULL);
+			qh_h2d(dMm, Mm, m * m * sizeof(complex double), stream);
+			qh_memset(dr, 0, m * sizeof(complex double), stream);
 			qh_gemm_sub_f64((int)m, 1, (int)m, 1, (const double *)dMm, (long)m, 0,
-				(const double *)(dY + k * m), 1, 0, (double *)dr, 1, 0, NULL);
-			residual = qh_norm2_c64((long)m, (const double *)dr, NULL);
+				(const double *)(dY + k * m), 1, 0, (double *)dr, 1, 0, stream);
+			residual = qh_norm2_c64((long)m, (const double *)dr, stream);
 			if (residual > res_tol) continue;
 		}
 		eigval[kept] = z;
@@ -134,6 +257,13 @@ static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params
 	free(Mm); free(Y); free(lam);
 	qh_dfree(dY); qh_dfree(dB); qh_dfree(dT); qh_dfree(dUt); qh_dfree(dVt);
 	return (int)kept;
+}
+
+/* Serving-loop entry: eigenvalues only, no residual filter. */
+int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
+		double rank_tol, size_t rank_sel_min, complex double *eigval, double *sigma, void *stream) {
+	return qh_beyn_process(m, l, NULL, NULL, dA0, dA1, z0, rank_tol, rank_sel_min, 0.0, eigval, NULL, NULL, sigma,
+		stream);
 }
 
 beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_function_M_inv_Vhat_t M_inv_Vhat,
@@ -171,7 +301,7 @@ beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_functi
 			QH_ENSURE(M(Mh, m, z, params) == 0, "Beyn: M callback failed");
 			qh_h2d(dM, Mh, m * m * sizeof(complex double), NULL);
 			if (mixed) {
-				qpms_hip_solve_mixed_dev(m, l, dM, dV, dX, 1e-15, 30, NULL);
+				qpms_hip_solve_mixed_dev(m, l, dM, dV, dX, 1e-17, 30, NULL);   /* refine to the FP64 floor */
 			} else {
 				qh_memset(dinfo, 0, sizeof(int), NULL);
 				qh_getrf_batched_f64((int)m, 1, (double *)dM, 0, dpiv, dperm, dinfo, NULL);
@@ -197,9 +327,9 @@ beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_functi
 	QH_MALLOC(sv_coarse, (l ? l : 1) * sizeof(double));
 	/* coarse pass first (its values become the error estimate), then fine */
 	qh_beyn_process(m, l, M, params, dA0c, dA1c, z0, rank_tol, rank_sel_min, res_tol,
-		r->eigval_err, NULL, NULL, sv_coarse);
+		r->eigval_err, NULL, NULL, sv_coarse, NULL);
 	const int K = qh_beyn_process(m, l, M, params, dA0, dA1, z0, rank_tol, rank_sel_min, res_tol,
-		r->eigval, r->eigvec, r->residuals, r->ranktest_SV);
+		r->eigval, r->eigvec, r->residuals, r->ranktest_SV, NULL);
 	for (size_t i = 0; i < l; ++i) r->eigval_err[i] -= r->eigval[i];   /* beyn.c:592 */
 	r->neig = (size_t)K;
 	free(sv_coarse);
@@ -254,6 +384,39 @@ beyn_result_t *qpms_scatsys_finite_find_eigenmodes(const qpms_scatsys_t *ss, qpm
 	beyn_contour_t *contour = beyn_contour_ellipse(omega_centre, omega_rr, omega_ri, contour_npoints);
 	struct qh_finite_beyn_param p = {ss};
 	beyn_result_t *r = beyn_solve(n, l, qh_finite_eval_ImTS, NULL, &p, contour, rank_tol, rank_sel_min, res_tol);
+	free(contour);
+	return r;
+}
+
+/* Periodic wrapper (scatsystem.c:2327-2370): M(omega) = I - T(omega) W(omega, k)
+ * through the Ewald builder at the adjusted eta. */
+struct qh_periodic_beyn_param { const qpms_scatsys_t *ss; const double *k; };
+
+static int qh_periodic_eval_ImTW(complex double *target, size_t m, complex double omega, void *params) {
+	const struct qh_periodic_beyn_param *p = params;
+	qpms_scatsys_at_omega_t *ssw = qpms_scatsys_at_omega(p->ss, omega);
+	QH_ENSURE(m == p->ss->fecv_size, "matrix size mismatch");
+	qpms_scatsys_at_omega_k_t sswk;
+	sswk.ssw = ssw;
+	sswk.k[0] = p->k[0]; sswk.k[1] = p->k[1]; sswk.k[2] = p->k[2];
+	sswk.eta = qpms_ss_adjusted_eta(p->ss, ssw->wavenumber, p->k);
+	qpms_scatsyswk_build_modeproblem_matrix_full(target, &sswk);
+	qpms_scatsys_at_omega_free(ssw);
+	return 0;
+}
+
+beyn_result_t *qpms_scatsys_periodic_find_eigenmodes(const qpms_scatsys_t *ss, const double k[3],
+		complex double omega_centre, double omega_rr, double omega_ri, size_t contour_npoints,
+		double rank_tol, size_t rank_sel_min, double res_tol) {
+	QH_ENSURE(ss->lattice_dimension > 0, "qpms_scatsys_finite_find_eigenmodes(): this method is applicable only to periodic systems.");
+	const size_t n = ss->fecv_size;
+	const char *sl = getenv("QPMS_HIP_BEYN_L");
+	size_t l = sl ? (size_t)strtoul(sl, NULL, 10) : 64;
+	if (l > n) l = n;
+	if (l > 64) l = 64;
+	beyn_contour_t *contour = beyn_contour_ellipse(omega_centre, omega_rr, omega_ri, contour_npoints);
+	struct qh_periodic_beyn_param p = {ss, k};
+	beyn_result_t *r = beyn_solve(n, l, qh_periodic_eval_ImTW, NULL, &p, contour, rank_tol, rank_sel_min, res_tol);
 	free(contour);
 	return r;
 }

@@ -121,7 +121,7 @@ static void qh_sph2cart(sph_t s, double *x, double *y, double *z) {
 	*z = s.r * cos(s.theta);
 }
 
-static double *qh_collect(PGen *g, size_t *n, int dims) {
+double *qh_collect(PGen *g, size_t *n, int dims) {
 	size_t cap = 1024, cnt = 0;
 	double *p;
 	QH_MALLOC(p, cap * dims * sizeof(double));
@@ -201,7 +201,7 @@ int ewald3_sigma_long(complex double *target, double *err, const qpms_ewald3_con
 /* ---------------------------------------------------------------------- */
 /* e32 (translations.c:926-1094)                                             */
 /* ---------------------------------------------------------------------- */
-static int qh_e32_bq_mode(void) {
+int qh_e32_bq_mode(void) {
 	const char *s = getenv("QPMS_HIP_E32_B_QMAX");
 	return (s && !strcmp(s, "corrected")) ? 1 : 0;
 }
@@ -260,12 +260,11 @@ static void qh_e32_device(const qpms_trans_calculator *c, int ntrip, const cart3
 	const qh_e3dev *e = qh_e3_get(e3);
 	if (parts & QPMS_EWALD_LONG_RANGE)
 		qh_launch_sigma_long(e3->lMax, ntrip, dtr, (int)nK, dK, eta, area, e->leg0, e->s1, e->s1off, dsl, NULL, NULL);
-	if (parts & QPMS_EWALD_SHORT_RANGE) {
-		/* triples with s = 0 sum without the origin, the others with it */
-		for (int t = 0; t < ntrip; ++t)
-			qh_launch_sigma_short(e3->lMax, shifts[t].z == 0, 1, dtr + 8 * t, flag0[t] ? (int)nR - 1 : (int)nR,
-				flag0[t] ? dR + 3 : dR, eta, e->leg0, e3->legendre_csphase, dss + 2 * nsc * t, NULL, NULL);
-	}
+	/* one launch for every triple: the kernel drops the lattice point equal
+	 * to the shift, i.e. the origin for s = 0 */
+	if (parts & QPMS_EWALD_SHORT_RANGE)
+		qh_launch_sigma_short(e3->lMax, all_xy, ntrip, dtr, (int)nR, dR, eta, e->leg0, e3->legendre_csphase, dss,
+			NULL, NULL);
 	double *kk;
 	QH_MALLOC(kk, 2 * ntrip * sizeof(double));
 	for (int t = 0; t < ntrip; ++t) { kk[2 * t] = creal(k); kk[2 * t + 1] = cimag(k); }

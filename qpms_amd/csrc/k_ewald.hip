@@ -66,9 +66,17 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
 #pragma unroll
   for (int i = 0; i < NSC; ++i) acc[i] = mk(0.0, 0.0);
   double aerr = 0.0;
+  // Every short-range term carries exp(-R^2 eta^2 + Re kappa^2), kappa = k / (2 eta):
+  // where that exponent is below -746 the factor is exactly 0 in double and
+  // so is the term; such points are skipped (the sum is unchanged bit for bit).
+  const double kap2re = (T[3] * T[3] - T[4] * T[4]) / (4.0 * eta * eta);
   for (int ip = lane; ip < npts; ip += 64) {
     const double rx = pts[3 * ip] - T[5], ry = pts[3 * ip + 1] - T[6], rz = pts[3 * ip + 2] - T[7];
     const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    // R = s: the lattice point coinciding with the particle (the origin for
+    // s = 0, translations.c:981-987) carries no short-range term.
+    if (r == 0.0) continue;
+    if (r * r * eta * eta - kap2re > 746.0) continue;
     cplx In[LE + 1];
     qew::sr_integrals<LE>(r, k, eta, c_weid, In);
     const cplx eb = cexpi(T[0] * rx + T[1] * ry + T[2] * rz);

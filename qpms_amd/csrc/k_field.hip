@@ -64,7 +64,7 @@ __global__ void k_planewave(int npart, const double *__restrict__ pos, double kx
   if (t >= npart * NE) return;
   const int p = t / NE, y = t % NE;
   int l = 1;
-  while ((l + 1) * (l + 2) - 1 <= y) ++l;
+  while ((l + 1) * (l + 1) - 1 <= y) ++l;  // y = l(l+1) + m - 1, |m| <= l
   const int m = y - l * (l + 1) + 1;
   // direction of Re k (cart2sph, vectors.h:134-140)
   const double kr = sqrt(kx * kx + ky * ky + kz * kz);

@@ -6,22 +6,29 @@
 // Reference call sites: scatsystem.c:2214 (LAPACKE_zgetrf), 2270 (zgetrs),
 // beyn.c:556-561 (zgetrf + zgetrs with l right-hand sides).
 //
-// Structure per panel of NBW = 32 columns:
-//   k_panel     one workgroup factors the tall panel column by column
-//               (pivot = first max |Re|+|Im|, as izamax), keeps a row
-//               permutation array up to date;
-//   k_laswp     applies the panel's row swaps to the columns left and right
-//               of the panel (one thread per column, coalesced rows);
-//   k_trsm_l    U12 = L11^{-1} A12 (one thread per column, L11 in LDS);
-//   k_gemm_sub  A22 -= L21 U12 — the O(n^3) part. FP64 uses
-//               v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32; the
-//               complex product is split into four real MFMA chains on
-//               de-interleaved LDS tiles.
+// LU structure (getrf_batched): outer blocks of LU_NB = 256 columns.
+//   k_tslu_*      factor a tall strip of PW columns (64 in FP32, 32 in
+//                 FP64) with tournament pivoting: candidate rows chosen per
+//                 chunk in LDS, merged pairwise, the winners moved to the top
+//                 (recorded as zgetrf-style interchanges), then L21 by a
+//                 triangular solve; no workgroup waits for another;
+//   k_laswp       applies the strip's row swaps to every other column;
+//   k_trsm        U = L^{-1} B for a PW-row block (one thread per column,
+//                 the unit-lower triangle in LDS);
+//   k_gemm_sub    C -= A B: the rank-PW updates inside an outer block and
+//                 the rank-256 trailing update A22 -= L21 U12 (the O(n^3)
+//                 part). FP64 uses v_mfma_f64_16x16x4f64, FP32
+//                 v_mfma_f32_16x16x4f32; the complex product is four real
+//                 MFMA chains on de-interleaved LDS tiles.
 // Every kernel takes a batch index in blockIdx.z with per-matrix strides, so
 // the Beyn quadrature nodes factor in one launch sequence.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <rocblas/rocblas.h>
 #include "qpms_internal.h"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
@@ -54,78 +61,6 @@ template <class T> __device__ __forceinline__ typename C2<T>::t cdiv(typename C2
   }
 }
 
-// ---------------------------------------------------------------------------
-// Panel factorisation: rows k0..n-1, columns k0..k0+w-1 of matrix z.
-// perm (if non-null) holds the current row permutation (perm[i] = original
-// row now at position i) and is updated with the panel's swaps.
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_panel(int n, int k0, int w, typename C2<T>::t *A, long lda, long sA,
-                                               int *ipiv, long sP, int *perm, int *info) {
-  typedef typename C2<T>::t cT;
-  A += (long)blockIdx.z * sA;
-  ipiv += (long)blockIdx.z * sP;
-  if (perm) perm += (long)blockIdx.z * sP;
-  info += blockIdx.z;
-  __shared__ T sval[256];
-  __shared__ int sidx[256];
-  __shared__ cT prow[NBW];
-  const int t = threadIdx.x;
-  for (int j = 0; j < w; ++j) {
-    const int col = k0 + j;
-    // pivot search: first index of the maximum |Re| + |Im|
-    T best = -1;
-    int bi = n;
-    for (int i = col + t; i < n; i += 256) {
-      cT a = A[(long)i * lda + col];
-      T v = fabs(a.x) + fabs(a.y);
-      if (v > best) { best = v; bi = i; }
-    }
-    sval[t] = best;
-    sidx[t] = bi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) {
-        T v2 = sval[t + s];
-        int i2 = sidx[t + s];
-        if (v2 > sval[t] || (v2 == sval[t] && i2 < sidx[t])) { sval[t] = v2; sidx[t] = i2; }
-      }
-      __syncthreads();
-    }
-    const int p = sidx[0];
-    const T pmax = sval[0];
-    __syncthreads();
-    if (t == 0) {
-      ipiv[col] = p + 1;
-      if (pmax == 0 && *info == 0) *info = col + 1;
-      if (perm && p != col) { int a = perm[col]; perm[col] = perm[p]; perm[p] = a; }
-    }
-    if (p != col && pmax != 0) {
-      for (int c = t; c < w; c += 256) {
-        cT a = A[(long)col * lda + k0 + c], b = A[(long)p * lda + k0 + c];
-        A[(long)col * lda + k0 + c] = b;
-        A[(long)p * lda + k0 + c] = a;
-      }
-    }
-    __syncthreads();
-    if (t < w) prow[t] = A[(long)col * lda + k0 + t];
-    __syncthreads();
-    if (pmax != 0) {
-      const cT d = prow[j];
-      for (int i = col + 1 + t; i < n; i += 256) {
-        cT *row = A + (long)i * lda + k0;
-        const cT l = cdiv<T>(row[j], d);
-        row[j] = l;
-        for (int c = j + 1; c < w; ++c) {
-          const cT u = prow[c];
-          row[c] = cT{row[c].x - (l.x * u.x - l.y * u.y), row[c].y - (l.x * u.y + l.y * u.x)};
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Row swaps of rows k0..k0+w-1 (pivots ipiv[k0..]) on columns outside the panel.
 template <class T>
 __global__ __launch_bounds__(256) void k_laswp(int n, int k0, int w, typename C2<T>::t *A, long lda, long sA,
@@ -149,10 +84,11 @@ __global__ __launch_bounds__(256) void k_laswp(int n, int k0, int w, typename C2
 
 // A12 <- L11^{-1} A12 with L11 unit lower triangular (w x w at (k0, k0)).
 // Generic form: rows r0..r0+w of B (ldb) against the triangle at L (ldl).
-template <class T, bool UNIT_LOWER>
+template <class T, bool UNIT_LOWER, int WM = NBW>
 __global__ __launch_bounds__(256) void k_trsm(int w, int ncols, const typename C2<T>::t *L, long ldl, long sL,
                                               typename C2<T>::t *B, long ldb, long sB) {
   typedef typename C2<T>::t cT;
+  constexpr int NBW = WM;
   L += (long)blockIdx.z * sL;
   B += (long)blockIdx.z * sB;
   __shared__ cT sl[NBW * NBW];
@@ -376,27 +312,450 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 // ===========================================================================
 // Host drivers
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Tall-strip panel: LU with tournament pivoting (CALU / TSLU, Grigori,
+// Demmel & Xiang 2011). The strip (rows c0..n-1, columns c0..c0+w-1,
+// w <= PW) is split into chunks of RPB rows:
+//   k_tslu_local  each workgroup runs partial-pivoting elimination on its
+//                 chunk in LDS only to choose w candidate rows, and writes
+//                 their indices and original values;
+//   k_tslu_merge  pairs of candidate sets are merged the same way, level by
+//                 level, until one set of w rows remains; the order in which
+//                 the last merge chose them is the pivot order;
+//   k_tslu_top    one workgroup per matrix moves those rows to the top as w
+//                 row interchanges (ipiv, perm), factors the w x w block
+//                 without further pivoting and sets info on a zero pivot;
+//   k_tslu_l21    every other strip row gets L21 = A21 U11^{-1}.
+// No workgroup ever waits for another one; each step is its own launch.
+// ---------------------------------------------------------------------------
+constexpr int PANEL_LDS = 64 * 1024;
+
+template <class T> struct PanelW;
+template <> struct PanelW<float> { static constexpr int W = 32; };
+template <> struct PanelW<double> { static constexpr int W = 32; };
+
+template <class T>
+static int panel_rows_per_block() { return PANEL_LDS / (PanelW<T>::W * (int)sizeof(typename C2<T>::t)); }
+
+// Per matrix: two candidate buffers of Gmax sets x PW rows (index + PW values).
+template <class T>
+struct PanelWS {
+  static size_t set_bytes() {
+    constexpr int PW = PanelW<T>::W;
+    return (size_t)PW * 4 + (size_t)PW * PW * sizeof(typename C2<T>::t);
+  }
+  static size_t bytes(int G) { return 2 * (size_t)G * ((set_bytes() + 255) / 256 * 256) + 256; }
+};
+
+// Partial-pivoting selection on X[nrows][PW] (LDS, eliminated in place):
+// the first min(w, nrows) pivot rows in order into sel[]. key[r] orders ties
+// (smaller key wins, as izamax's first maximum). Per column: a wave-shuffle
+// arg-max, one LDS combine of the four waves, the scaling of the column and
+// the rank-1 update of the unused rows (three barriers).
+template <class T>
+__device__ int gepp_select(typename C2<T>::t *X, int nrows, int w, const int *key, int *sel, int *used,
+                           T (*wv)[4], int (*wr)[4]) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int r = t; r < nrows; r += 256) used[r] = 0;
+  __syncthreads();
+  const int ns = min(w, nrows);
+  for (int j = 0; j < ns; ++j) {
+    const int par = j & 1;
+    T best = -1;
+    int bk = 0x7fffffff, br = -1;
+    for (int r = t; r < nrows; r += 256) {
+      if (used[r]) continue;
+      const cT v = X[r * PW + j];
+      const T a = fabs(v.x) + fabs(v.y);
+      if (a > best || (a == best && key[r] < bk)) { best = a; bk = key[r]; br = r; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T v2 = __shfl_xor(best, off);
+      const int k2 = __shfl_xor(bk, off), r2 = __shfl_xor(br, off);
+      if (v2 > best || (v2 == best && k2 < bk)) { best = v2; bk = k2; br = r2; }
+    }
+    if (lane == 0) { wv[par][wave] = best; wr[par][wave] = br; }
+    __syncthreads();
+    T pv = -1;
+    int p = -1, pk = 0x7fffffff;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const T v = wv[par][q];
+      const int r = wr[par][q];
+      const int k = (r >= 0) ? key[r] : 0x7fffffff;
+      if (v > pv || (v == pv && k < pk)) { pv = v; p = r; pk = k; }
+    }
+    if (p < 0) {   // only NaNs left: the first unused row
+      p = 0;
+      while (p < nrows - 1 && used[p]) ++p;
+    }
+    if (pv > 0) {
+      const cT d = X[p * PW + j];
+      for (int r = t; r < nrows; r += 256)
+        if (!used[r] && r != p) X[r * PW + j] = cdiv<T>(X[r * PW + j], d);
+    }
+    __syncthreads();
+    if (t == 0) { sel[j] = p; used[p] = 1; }
+    if (pv > 0) {
+      const int nc = w - j - 1;
+      for (int e = t; e < nrows * nc; e += 256) {
+        const int r = e / nc, c = j + 1 + e % nc;
+        if (!used[r] && r != p) {
+          const cT l = X[r * PW + j], u = X[p * PW + c];
+          cT &x = X[r * PW + c];
+          x = cT{x.x - (l.x * u.x - l.y * u.y), x.y - (l.x * u.y + l.y * u.x)};
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return ns;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tslu_local(int n, int c0, int w, int RPB, const typename C2<T>::t *A, long lda,
+                                                    long sA, char *ws, long sW, long setb) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
+  extern __shared__ char smem_raw[];
+  cT *X = reinterpret_cast<cT *>(smem_raw);
+  __shared__ int key[512], used[512], sel[PW], wr[2][4];
+  __shared__ T wv[2][4];
+  const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x;
+  A += (long)z * sA;
+  char *set = ws + (long)z * sW + (long)g * setb;
+  int *oidx = reinterpret_cast<int *>(set);
+  cT *oval = reinterpret_cast<cT *>(set + PW * 4);
+  const int m = n - c0, r0 = g * RPB, nr = max(0, min(RPB, m - r0));
+  for (int e = t; e < nr * PW; e += 256) {
+    const int r = e / PW, c = e % PW;
+    X[e] = (c < w) ? A[(long)(c0 + r0 + r) * lda + c0 + c] : cT{0, 0};
+  }
+  for (int r = t; r < nr; r += 256) key[r] = r0 + r;
+  __syncthreads();
+  const int ns = gepp_select<T>(X, nr, w, key, sel, used, wv, wr);
+  for (int i = t; i < PW; i += 256) oidx[i] = (i < ns) ? r0 + sel[i] : -1;
+  for (int e = t; e < ns * PW; e += 256) {
+    const int i = e / PW, c = e % PW;
+    oval[e] = (c < w) ? A[(long)(c0 + r0 + sel[i]) * lda + c0 + c] : cT{0, 0};
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tslu_merge(int w, int nin, const char *wsin, char *wsout, long sW,
+                                                    long setb) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
+  extern __shared__ char smem_raw[];
+  cT *X = reinterpret_cast<cT *>(smem_raw);   // [2 PW][PW]
+  __shared__ int key[2 * PW], used[2 * PW], sel[PW], src[2 * PW], wr[2][4];
+  __shared__ T wv[2][4];
+  __shared__ int nrows_s;
+  const int z = blockIdx.z, b = blockIdx.x, t = threadIdx.x;
+  const char *in = wsin + (long)z * sW;
+  char *out = wsout + (long)z * sW + (long)b * setb;
+  if (t == 0) {
+    int nr = 0;
+    for (int q = 0; q < 2; ++q) {
+      const int si = 2 * b + q;
+      if (si >= nin) break;
+      const int *idx = reinterpret_cast<const int *>(in + (long)si * setb);
+      for (int i = 0; i < PW; ++i)
+        if (idx[i] >= 0) { key[nr] = idx[i]; src[nr] = si * PW + i; ++nr; }
+    }
+    nrows_s = nr;
+  }
+  __syncthreads();
+  const int nr = nrows_s;
+  for (int e = t; e < nr * PW; e += 256) {
+    const int r = e / PW, c = e % PW;
+    const int si = src[r] / PW, i = src[r] % PW;
+    X[e] = reinterpret_cast<const cT *>(in + (long)si * setb + PW * 4)[i * PW + c];
+  }
+  __syncthreads();
+  const int ns = gepp_select<T>(X, nr, w, key, sel, used, wv, wr);
+  int *oidx = reinterpret_cast<int *>(out);
+  cT *oval = reinterpret_cast<cT *>(out + PW * 4);
+  for (int i = t; i < PW; i += 256) oidx[i] = (i < ns) ? key[sel[i]] : -1;
+  for (int e = t; e < ns * PW; e += 256) {
+    const int i = e / PW, c = e % PW;
+    const int si = src[sel[i]] / PW, ii = src[sel[i]] % PW;
+    oval[e] = reinterpret_cast<const cT *>(in + (long)si * setb + PW * 4)[ii * PW + c];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tslu_top(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
+                                                  const char *wsfin, long sW, int *ipiv, long sP, int *perm,
+                                                  int *info) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
+  __shared__ int tgt[PW], pos_of[2 * PW], row_of[2 * PW], ids[2 * PW], swp[PW];
+  __shared__ cT U[PW * PW];
+  __shared__ int nids;
+  const int z = blockIdx.z, t = threadIdx.x;
+  A += (long)z * sA;
+  ipiv += (long)z * sP;
+  if (perm) perm += (long)z * sP;
+  info += z;
+  const int *fidx = reinterpret_cast<const int *>(wsfin + (long)z * sW);
+  if (t < w) tgt[t] = fidx[t];
+  __syncthreads();
+  for (int i = 0; i < w; ++i)
+    if (tgt[i] < 0 || tgt[i] >= n - c0) {   // cannot happen for n - c0 >= w; never index with it
+      if (t == 0) *info = -1;
+      return;
+    }
+  // interchange sequence: strip rows involved are 0..w-1 and the targets
+  if (t == 0) {
+    int k = 0;
+    for (int i = 0; i < w; ++i) { ids[k] = i; pos_of[k] = i; row_of[k] = i; ++k; }
+    for (int i = 0; i < w; ++i) {
+      int f = -1;
+      for (int q = 0; q < k; ++q) if (ids[q] == tgt[i]) { f = q; break; }
+      if (f < 0) { ids[k] = tgt[i]; pos_of[k] = tgt[i]; row_of[k] = tgt[i]; ++k; }
+    }
+    // ids[q]: original strip row; pos_of[q]: its current position
+    for (int i = 0; i < w; ++i) {
+      int qt = 0;
+      for (int q = 0; q < k; ++q) if (ids[q] == tgt[i]) { qt = q; break; }
+      const int p = pos_of[qt];
+      swp[i] = p;
+      if (p != i) {
+        int qi = 0;   // entry whose current position is i
+        for (int q = 0; q < k; ++q) if (pos_of[q] == i) { qi = q; break; }
+        pos_of[qi] = p;
+        pos_of[qt] = i;
+      }
+    }
+    nids = k;
+  }
+  __syncthreads();
+  for (int i = 0; i < w; ++i) {
+    const int p = swp[i];
+    if (p != i) {
+      for (int c = t; c < w; c += 256) {
+        const cT a = A[(long)(c0 + i) * lda + c0 + c], b = A[(long)(c0 + p) * lda + c0 + c];
+        A[(long)(c0 + i) * lda + c0 + c] = b;
+        A[(long)(c0 + p) * lda + c0 + c] = a;
+      }
+    }
+    if (t == 0) {
+      ipiv[c0 + i] = c0 + p + 1;
+      if (perm && p != i) { const int a = perm[c0 + i]; perm[c0 + i] = perm[c0 + p]; perm[c0 + p] = a; }
+    }
+    __syncthreads();
+  }
+  // unpivoted LU of the top w x w block
+  for (int e = t; e < w * w; e += 256) U[(e / w) * PW + e % w] = A[(long)(c0 + e / w) * lda + c0 + e % w];
+  __syncthreads();
+  for (int j = 0; j < w; ++j) {
+    const cT d = U[j * PW + j];
+    const bool zero = (d.x == 0 && d.y == 0);
+    if (t == 0 && zero && *info == 0) *info = c0 + j + 1;
+    if (!zero) {
+      for (int r = j + 1 + t; r < w; r += 256) U[r * PW + j] = cdiv<T>(U[r * PW + j], d);
+      __syncthreads();
+      const int nc = w - j - 1;
+      for (int e = t; e < nc * nc; e += 256) {
+        const int r = j + 1 + e / nc, c = j + 1 + e % nc;
+        const cT l = U[r * PW + j], u = U[j * PW + c];
+        cT &x = U[r * PW + c];
+        x = cT{x.x - (l.x * u.x - l.y * u.y), x.y - (l.x * u.y + l.y * u.x)};
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < w * w; e += 256) A[(long)(c0 + e / w) * lda + c0 + e % w] = U[(e / w) * PW + e % w];
+  (void)nids; (void)row_of;
+}
+
+// L21 = A21 U11^{-1}: one thread per strip row below the top block.
+template <class T>
+__global__ __launch_bounds__(256) void k_tslu_l21(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
+  __shared__ cT U[PW * PW];
+  A += (long)blockIdx.z * sA;
+  for (int e = threadIdx.x; e < w * w; e += 256) U[(e / w) * PW + e % w] = A[(long)(c0 + e / w) * lda + c0 + e % w];
+  __syncthreads();
+  const int r = c0 + w + blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  cT *row = A + (long)r * lda + c0;
+  cT x[PW];
+#pragma unroll
+  for (int i = 0; i < PW; ++i) {
+    if (i < w) {
+      cT v = row[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) {
+        const cT u = U[k * PW + i];
+        v = cT{v.x - (x[k].x * u.x - x[k].y * u.y), v.y - (x[k].x * u.y + x[k].y * u.x)};
+      }
+      const cT d = U[i * PW + i];
+      x[i] = (d.x == 0 && d.y == 0) ? v : cdiv<T>(v, d);
+      row[i] = x[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Trailing update C -= A B of the LU (M x N x K = n-k x n-k x 256): a plain
+// complex GEMM, handed to rocBLAS (cgemm / zgemm on MFMA, FP32 or FP64
+// accumulation) when it is large; k_gemm_sub otherwise, or everywhere with
+// QPMS_HIP_GEMM=own. Row-major C = A B is column-major C^T = B^T A^T.
+// ---------------------------------------------------------------------------
+static bool use_rocblas() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("QPMS_HIP_GEMM");
+    v = !(e && !strcmp(e, "own"));
+  }
+  return v == 1;
+}
+
+static rocblas_handle blas_for(hipStream_t s) {
+  static std::mutex mu;
+  static hipStream_t keys[16];
+  static rocblas_handle vals[16];
+  static int n = 0;
+  std::lock_guard<std::mutex> g(mu);
+  for (int i = 0; i < n; ++i) if (keys[i] == s) return vals[i];
+  if (n == 16) qpms_hip_die(__FILE__, __LINE__, __func__, "too many streams for the GEMM handle table");
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_create_handle failed");
+  if (rocblas_set_stream(h, s) != rocblas_status_success)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_set_stream failed");
+  keys[n] = s;
+  vals[n] = h;
+  return vals[n++];
+}
+
+template <class T>
+static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::t *A, long lda, long sA,
+                          const typename C2<T>::t *B, long ldb, long sB, typename C2<T>::t *C, long ldc, long sC,
+                          hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  if (!use_rocblas() || (long)M * N < 512L * 512) {
+    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0, s, M, N, K, A,
+                       lda, sA, B, ldb, sB, C, ldc, sC);
+    return;
+  }
+  rocblas_handle h = blas_for(s);
+  rocblas_status st;
+  if (sizeof(T) == 4) {
+    const rocblas_float_complex al = {-1.f, 0.f}, be = {1.f, 0.f};
+    st = rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &al,
+                                       (const rocblas_float_complex *)B, (rocblas_int)ldb, sB,
+                                       (const rocblas_float_complex *)A, (rocblas_int)lda, sA, &be,
+                                       (rocblas_float_complex *)C, (rocblas_int)ldc, sC, batch);
+  } else {
+    const rocblas_double_complex al = {-1.0, 0.0}, be = {1.0, 0.0};
+    st = rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &al,
+                                       (const rocblas_double_complex *)B, (rocblas_int)ldb, sB,
+                                       (const rocblas_double_complex *)A, (rocblas_int)lda, sA, &be,
+                                       (rocblas_double_complex *)C, (rocblas_int)ldc, sC, batch);
+  }
+  if (st != rocblas_status_success)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "rocBLAS GEMM failed (status %d)", (int)st);
+}
+
+constexpr int LU_NB = 256;   // outer block: K of the trailing GEMM
+
+// Workspace bytes for getrf_batched<T>(n, batch).
+template <class T>
+static size_t getrf_ws_bytes(int n, int batch) {
+  const int RPB = panel_rows_per_block<T>();
+  const int G = (n + RPB - 1) / RPB;
+  return (size_t)batch * ((PanelWS<T>::bytes(G) + 255) / 256 * 256);
+}
+
+// Blocked right-looking LU with tournament-pivoted panels of `batch` n x n
+// matrices (stride sA), in place. Outer blocks of LU_NB columns; inside a
+// block the columns are factorised PW at a time (k_tslu_*), the block's
+// remaining columns updated by TRSM + GEMM (K = PW); then U12 = L11^{-1} A12
+// blocked by PW and the trailing A22 -= L21 U12 (K = LU_NB).
+// ipiv (1-based, stride sP) lists the row interchanges as zgetrf's does;
+// perm (optional) tracks P; info[z] gets the first zero pivot. ws:
+// getrf_ws_bytes<T>(n, batch) bytes, or nullptr for a library-owned buffer
+// (single-stream callers only).
 template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
-                          int *perm, int *dinfo, hipStream_t s) {
+                          int *perm, int *dinfo, hipStream_t s, char *ws = nullptr) {
+  typedef typename C2<T>::t cT;
+  constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
+  if (n <= 0 || batch <= 0) return;
+  const int RPB = panel_rows_per_block<T>();
+  const int Gmax = (n + RPB - 1) / RPB;
+  const long sW = (long)((PanelWS<T>::bytes(Gmax) + 255) / 256 * 256);
+  const long setb = (long)((PanelWS<T>::set_bytes() + 255) / 256 * 256);
+  if (!ws) {
+    static char *own = nullptr;
+    static size_t own_bytes = 0;
+    const size_t need = (size_t)sW * batch;
+    if (need > own_bytes) {
+      if (own) { HIPCHK(hipStreamSynchronize(s)); qh_dfree(own); }
+      own = (char *)qh_dmalloc(need);
+      own_bytes = need;
+    }
+    ws = own;
+  }
+  char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
   if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
-  for (int k0 = 0; k0 < n; k0 += NBW) {
-    const int w = (n - k0 < NBW) ? n - k0 : NBW;
-    hipLaunchKernelGGL(k_panel<T>, dim3(1, 1, batch), b256, 0, s, n, k0, w, A, lda, sA, ipiv, sP, perm, dinfo);
-    const int nout = n - w;
-    if (nout > 0)
-      hipLaunchKernelGGL(k_laswp<T>, dim3((nout + 255) / 256, 1, batch), b256, 0, s, n, k0, w, A, lda, sA, ipiv, sP);
-    const int nr = n - k0 - w;
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int nb = min(LU_NB, n - k0);
+    for (int j0 = 0; j0 < nb; j0 += PW) {
+      const int w = min(PW, nb - j0), c0 = k0 + j0;
+      // tournament-pivoted panel of the strip rows c0..n-1
+      const int G = (n - c0 + RPB - 1) / RPB;
+      hipLaunchKernelGGL(k_tslu_local<T>, dim3(G, 1, batch), b256, PANEL_LDS, s, n, c0, w, RPB, A, lda, sA, bufA, sW,
+                         setb);
+      char *cur = bufA, *nxt = bufB;
+      for (int nsets = G; nsets > 1; nsets = (nsets + 1) / 2) {
+        hipLaunchKernelGGL(k_tslu_merge<T>, dim3((nsets + 1) / 2, 1, batch), b256,
+                           (size_t)2 * PW * PW * sizeof(cT), s, w, nsets, cur, nxt, sW, setb);
+        char *tmp = cur; cur = nxt; nxt = tmp;
+      }
+      hipLaunchKernelGGL(k_tslu_top<T>, dim3(1, 1, batch), b256, 0, s, n, c0, w, A, lda, sA, cur, sW, ipiv, sP, perm,
+                         dinfo);
+      if (n - c0 - w > 0)
+        hipLaunchKernelGGL(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, s, n, c0, w, A, lda, sA);
+      // the strip's interchanges on every other column
+      if (n - w > 0)
+        hipLaunchKernelGGL(k_laswp<T>, dim3((n - w + 255) / 256, 1, batch), b256, 0, s, n, c0, w, A, lda, sA, ipiv, sP);
+      // rest of this outer block: U part by TRSM, then the rank-w update
+      const int rest = k0 + nb - (c0 + w);
+      if (rest > 0) {
+        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, s, w, rest,
+                           A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);
+        const int mr = n - c0 - w;
+        if (mr > 0)
+          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, s, mr,
+                             rest, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA,
+                             A + (long)(c0 + w) * lda + c0 + w, lda, sA);
+      }
+    }
+    const int nr = n - k0 - nb;
     if (nr > 0) {
-      typename C2<T>::t *A11 = A + (long)k0 * lda + k0;
-      typename C2<T>::t *A12 = A + (long)k0 * lda + k0 + w;
-      typename C2<T>::t *A21 = A + (long)(k0 + w) * lda + k0;
-      typename C2<T>::t *A22 = A + (long)(k0 + w) * lda + k0 + w;
-      hipLaunchKernelGGL((k_trsm<T, true>), dim3((nr + 255) / 256, 1, batch), b256, 0, s, w, nr, A11, lda, sA, A12,
-                         lda, sA);
-      hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nr + GT - 1) / GT, (nr + GT - 1) / GT, batch), b256, 0, s, nr, nr, w,
-                         A21, lda, sA, A12, lda, sA, A22, lda, sA);
+      // U12 = L11^{-1} A12, PW rows at a time
+      for (int j0 = 0; j0 < nb; j0 += PW) {
+        const int w = min(PW, nb - j0), c0 = k0 + j0;
+        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((nr + 255) / 256, 1, batch), b256, 0, s, w, nr,
+                           A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA);
+        const int below = nb - j0 - w;
+        if (below > 0)
+          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nr + GT - 1) / GT, (below + GT - 1) / GT, batch), b256, 0, s, below,
+                             nr, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA,
+                             A + (long)(c0 + w) * lda + k0 + nb, lda, sA);
+      }
+      // A22 -= L21 U12
+      trailing_gemm<T>(nr, nr, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA, A + (long)k0 * lda + k0 + nb, lda,
+                       sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s);
     }
   }
   HIPCHK(hipGetLastError());
@@ -481,6 +840,19 @@ extern "C" int qh_launch_block_tw(size_t nr, size_t L, const double *T, const do
 extern "C" int qh_getrf_batched_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info,
                                     void *stream) {
   getrf_batched<double>(n, batch, (v2d *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream));
+  return 0;
+}
+extern "C" size_t qh_getrf_ws_bytes(int prec, int n, int batch) {
+  return prec == 32 ? getrf_ws_bytes<float>(n, batch) : getrf_ws_bytes<double>(n, batch);
+}
+extern "C" int qh_getrf_batched_ws_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info, void *ws,
+                                       void *stream) {
+  getrf_batched<double>(n, batch, (v2d *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws);
+  return 0;
+}
+extern "C" int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info, void *ws,
+                                       void *stream) {
+  getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws);
   return 0;
 }
 extern "C" int qh_getrs_batched_f64(int n, int nrhs, int batch, const double *LU, long sA, const int *perm,
@@ -583,7 +955,11 @@ extern "C" int qpms_hip_solve_mixed_dev(size_t n_, size_t nrhs_, const _Complex 
     const double nx = qh_norm2_c64(nb, (const double *)X_dev, s);
     const double rel = nr / (normA * nx + normB * 1e-300);
     if (rel <= tol) { ok = 1; ++it; break; }
-    if (!(rel < 0.5 * prev)) break;  // stalled
+    if (!(rel < 0.5 * prev)) {   // stalled: accept at the FP64 rounding floor
+      ok = (rel <= 1e-14);
+      ++it;
+      break;
+    }
     prev = rel;
   }
   qh_dfree(A32); qh_dfree(ipiv); qh_dfree(perm); qh_dfree(dinfo); qh_dfree(R); qh_dfree(R32); qh_dfree(W32);

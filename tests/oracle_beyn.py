@@ -125,3 +125,57 @@ def match_sets(a, b):
     if len(a) == 0:
         return 0.0
     return max(np.min(np.abs(b - x)) / max(abs(x), 1e-300) for x in a)
+
+
+def _orient(o, rre, rim):
+    """beyn.c:127-147"""
+    return {3: (-1j, rim, rre), 1: (1j, rim, rre), 0: (1, rre, rim), 2: (-1, rre, rim)}[o]
+
+
+def contour_halfellipse(centre, rre, rim, n, o):
+    """beyn.c:120-163"""
+    f, l, h = _orient(o, rre, rim)
+    nline = n // 2
+    narc = n - nline
+    t = (np.arange(narc) + 0.5) * np.pi / narc
+    z = [centre + f * (np.cos(t) * l + 1j * np.sin(t) * h)]
+    dz = [f * (-l * np.sin(t) + 1j * h * np.cos(t)) * (np.pi / narc)]
+    t = 0.5 * (1 - nline) + np.arange(nline)
+    z.append(centre + f * t * 2 * l / nline)
+    dz.append(np.full(nline, f * 2 * l / nline, complex))
+    return np.concatenate(z), np.concatenate(dz)
+
+
+def contour_kidney(centre, rre, rim, rounding, n, o):
+    """beyn.c:165-248: the sampling of a rounded half-ellipse by arc length."""
+    f, l, h = _orient(o, rre, rim)
+    y = rounding * h
+    x = np.sqrt((h - y) ** 2 - y ** 2)
+    al = np.arcsin(y / (h - y))
+    ar = l / h
+    tmax = 2 * (x + (np.pi / 2 + al) * y + (np.pi / 2 - al) * h)
+    dt = tmax / n
+    z, dz = [], []
+    b1 = x
+    b2 = b1 + (np.pi / 2 + al) * y
+    b3 = b2 + (np.pi - 2 * al) * h
+    b4 = b3 + (np.pi / 2 + al) * y
+    i = 0
+    while i * dt <= b1:
+        t = i * dt
+        z.append(centre + t * ar * f); dz.append(dt * ar * f); i += 1
+    while i * dt < b2:
+        phi = (i * dt - b1) / y - np.pi / 2
+        z.append(centre + (ar * (x + y * np.cos(phi)) + y * (1 + np.sin(phi)) * 1j) * f)
+        dz.append(dt * (-ar * np.sin(phi) + np.cos(phi) * 1j) * f); i += 1
+    while i * dt < b3:
+        phi = (i * dt - b2) / h + al
+        z.append(centre + (ar * h * np.cos(phi) + h * np.sin(phi) * 1j) * f)
+        dz.append(dt * (-ar * np.sin(phi) + np.cos(phi) * 1j) * f); i += 1
+    while i * dt < b4:
+        phi = (i * dt - b3) / y + np.pi - al
+        z.append(centre + (ar * (-x + y * np.cos(phi)) + y * (1 + np.sin(phi)) * 1j) * f)
+        dz.append(dt * (-ar * np.sin(phi) + np.cos(phi) * 1j) * f); i += 1
+    while i < n:
+        z.append(centre + (i * dt - b4 - x) * ar * f); dz.append(dt * ar * f); i += 1
+    return np.array(z), np.array(dz)

@@ -1,5 +1,5 @@
-"""C ABI: the shared library loads and exports every function that
-include/qpms_hip.h declares (no compute calls: runs without a GPU)."""
+"""C ABI: the shared library loads and exports every function that the
+headers under include/ declare (no compute calls: runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -8,7 +8,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    src = open(os.path.join(ROOT, "include", "qpms_hip.h")).read()
+    inc = os.path.join(ROOT, "include")
+    src = "\n".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     src = re.sub(r"static inline[^{]*\{[^}]*\}", "", src)
     names = set()

@@ -17,9 +17,16 @@ def inside_ellipse(z, c, rre, rim):
     return ((z.real - c.real) / rre) ** 2 + ((z.imag - c.imag) / rim) ** 2 < 1
 
 
-def check_against_oracle(res, ev, vecs, S, tol_ev=1e-9):
+def check_against_oracle(res, ev, vecs, S, tol_ev=1e-9, inside=None):
+    """Eigenvalues inside the contour agree to tol_ev; the ones the filters let
+    through outside it (poorly resolved by the quadrature, so sensitive to
+    rounding) to 1e-6."""
     assert res.neig == len(ev)
-    assert ob.match_sets(res.eigval, ev) < tol_ev and ob.match_sets(ev, res.eigval) < tol_ev
+    g_in = res.eigval[inside(res.eigval)] if inside else res.eigval
+    o_in = ev[inside(ev)] if inside else ev
+    assert len(g_in) == len(o_in)
+    assert ob.match_sets(g_in, o_in) < tol_ev and ob.match_sets(o_in, g_in) < tol_ev
+    assert ob.match_sets(res.eigval, ev) < 1e-6 and ob.match_sets(ev, res.eigval) < 1e-6
     l = len(S)
     assert np.max(np.abs(res.ranktest_SV - S)) <= 1e-10 * S[0] * l
     for k in range(res.neig):
@@ -38,7 +45,7 @@ def test_loaded_string(gpu, monkeypatch, lu_mode):
     res = qpms_amd.beyn_solve(Mf, m, l, c, r, r, n, 1e-4, 1, 1e-4, Vhat=V)
     z, dz = ob.contour_ellipse(c, r, r, n)
     ev, vecs, _, S, _ = ob.beyn_solve(Mf, m, V, z, dz, c, 1e-4, 1, 1e-4)
-    check_against_oracle(res, ev, vecs, S)
+    check_against_oracle(res, ev, vecs, S, inside=lambda z: inside_ellipse(z, c, r, r))
     assert np.all(res.residuals <= 1e-4)
     truth = ob.loaded_string_eigs(m)
     t_in = truth[inside_ellipse(truth, c, r, r)]
@@ -55,7 +62,7 @@ def test_quadratic_polynomial(gpu):
     res = qpms_amd.beyn_solve(Mf, dim, l, 0, 0.3, 0.3, n, 1e-4, 1, 1e-4, Vhat=V)
     z, dz = ob.contour_ellipse(0, 0.3, 0.3, n)
     ev, vecs, _, S, _ = ob.beyn_solve(Mf, dim, V, z, dz, 0, 1e-4, 1, 1e-4)
-    check_against_oracle(res, ev, vecs, S, tol_ev=1e-8)
+    check_against_oracle(res, ev, vecs, S, tol_ev=1e-8, inside=lambda z: np.abs(z) < 0.29)
     t_in = w[np.abs(w) < 0.29]
     e_in = res.eigval[np.abs(res.eigval) < 0.29]
     assert len(e_in) == len(t_in)
@@ -72,7 +79,7 @@ def test_no_residual_filter_keeps_rank(gpu):
     z, dz = ob.contour_ellipse(150 + 2j, 148, 148, 50)
     ev, vecs, _, S, _ = ob.beyn_solve(Mf, m, V, z, dz, 150 + 2j, 1e-4, 1, 0.0)
     assert res.neig == len(ev) == int(np.sum(S > 1e-4))
-    check_against_oracle(res, ev, vecs, S, tol_ev=1e-8)
+    check_against_oracle(res, ev, vecs, S, inside=lambda z: inside_ellipse(z, 150 + 2j, 148, 148))
 
 
 def test_seeded_probe_is_deterministic(gpu):

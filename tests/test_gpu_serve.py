@@ -1,0 +1,118 @@
+"""GPU: the serving loop (include/qpms_hip_serve.h) against the plain FP64
+drop-in path (the public scattering-system API, one request at a time) on a
+small mixed stream: plane-wave solves of several sizes (batched and
+unbatched), 2D-periodic determinants and Beyn contours. Each result must
+agree with the plain path (extinction and log-determinant to 1e-9 relative,
+Beyn singular values to 1e-9 of the largest) and be the same when the request
+runs alone as when it runs in the mix (1e-12)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EV = 1.602176634e-19 / 1.054571817e-34
+
+
+def grid(N, pitch, rng):
+    side = int(np.ceil(np.sqrt(N)))
+    g = np.array([(i, j) for i in range(side) for j in range(side)][:N], float)
+    g -= g.mean(axis=0)
+    xy = (g + rng.uniform(-0.1, 0.1, size=g.shape)) * pitch
+    return np.column_stack([xy, np.zeros(N)]), side * pitch
+
+
+def requests():
+    from qpms_amd import serve as sv
+    rng = np.random.default_rng(11)
+    out = []
+    for N, lMax in [(4, 2), (9, 2), (16, 3), (12, 2), (25, 3), (6, 4), (50, 4), (20, 2), (3, 1), (30, 2)]:
+        pos, _ = grid(N, 400e-9, rng)
+        out.append(sv.Req(sv.KIND_SCATTER, pos, lMax, rng.uniform(30e-9, 80e-9),
+                          complex(rng.uniform(-20, -5), rng.uniform(0.5, 2)), 1.52 ** 2,
+                          rng.uniform(1.5, 2.5) * EV, kdir=(0, 0, 1), E=(1, 0, 0)))
+    for N, lMax in [(2, 2), (4, 2), (3, 3)]:
+        pos, a = grid(N, 400e-9, rng)
+        out.append(sv.Req(sv.KIND_PERIODIC, pos, lMax, 50e-9, -12 + 1j, 1.52 ** 2, 2.0 * EV,
+                          lattice=((a, 0), (0, a)), beta=tuple(rng.uniform(-np.pi / a, np.pi / a, 2))))
+    for N, lMax in [(4, 2), (9, 2)]:
+        pos, _ = grid(N, 400e-9, rng)
+        out.append(sv.Req(sv.KIND_BEYN, pos, lMax, 70e-9, -12 + 1j, 1.52 ** 2, 2.05 * EV, beyn_rRe=0.05 * EV,
+                          beyn_rIm=0.005 * EV, beyn_nodes=16, beyn_l=16, beyn_seed=7 + N))
+    return out
+
+
+def drain(srv):
+    got = {}
+    while srv.pending():
+        srv.progress(wait=True)
+        for r in srv.poll():
+            got[r.id] = r
+    return got
+
+
+def close(a, b, kind, tol):
+    from qpms_amd import serve as sv
+    assert a.status == 0 and b.status == 0, (a.message, b.message)
+    assert a.dim == b.dim
+    if kind == sv.KIND_SCATTER:
+        assert abs(a.sigma_ext - b.sigma_ext) <= tol * abs(b.sigma_ext), (a.sigma_ext, b.sigma_ext)
+    elif kind == sv.KIND_PERIODIC:
+        assert abs(a.logabsdet - b.logabsdet) <= tol * max(1.0, abs(b.logabsdet)), (a.logabsdet, b.logabsdet)
+        d = (a.argdet - b.argdet + np.pi) % (2 * np.pi) - np.pi
+        assert abs(d) <= tol * max(1.0, abs(b.logabsdet)) + 1e-12
+    else:
+        sa, sb = np.array(a.sv[:16]), np.array(b.sv[:16])
+        assert np.max(np.abs(sa - sb)) <= tol * sb[0], (sa, sb)
+
+
+def test_serving_matches_plain_alone_and_mix(gpu):
+    from qpms_amd import serve as sv
+    reqs = requests()
+    plain = [sv.request_plain(r) for r in reqs]
+    srv = sv.Server(arena_gib=16, nstreams=4, mixed=True)
+    ids = [srv.submit(r) for r in reqs]
+    mix = drain(srv)
+    assert sorted(mix) == sorted(ids)
+    alone = []
+    for r in reqs:
+        rid = srv.submit(r)
+        alone.append(drain(srv)[rid])
+    srv.close()
+    for r, p, i, a in zip(reqs, plain, ids, alone):
+        close(mix[i], p, r.kind, 1e-9)
+        close(mix[i], a, r.kind, 1e-12)
+        if r.kind == sv.KIND_SCATTER:
+            assert mix[i].rel_residual < 1e-14
+
+
+def test_serving_fp64_mode_matches_plain(gpu):
+    from qpms_amd import serve as sv
+    reqs = [r for r in requests() if r.kind == sv.KIND_SCATTER][:6]
+    plain = [sv.request_plain(r) for r in reqs]
+    srv = sv.Server(arena_gib=8, mixed=False)
+    ids = [srv.submit(r) for r in reqs]
+    got = drain(srv)
+    srv.close()
+    for r, p, i in zip(reqs, plain, ids):
+        close(got[i], p, r.kind, 1e-10)
+
+
+def test_admission_by_footprint(gpu):
+    """A small arena: requests that never fit are refused at submit; the
+    others run, at most as many at a time as fit, and all finish."""
+    from qpms_amd import serve as sv
+    rng = np.random.default_rng(3)
+    srv = sv.Server(arena_gib=0.25, nstreams=4, mixed=True, batch_max_dim=512)
+    big, _ = grid(400, 400e-9, rng)   # L = 400 * 48 at lMax 4: 5.9 GB, never fits
+    with pytest.raises(MemoryError):
+        srv.submit(sv.Req(sv.KIND_SCATTER, big, 4, 50e-9, -12 + 1j, 2.31, 2 * EV))
+    reqs = []
+    for N in [40, 44, 48, 52, 20, 8]:
+        pos, _ = grid(N, 400e-9, rng)
+        reqs.append(sv.Req(sv.KIND_SCATTER, pos, 3, 50e-9, -12 + 1j, 2.31, 2 * EV))
+    ids = [srv.submit(r) for r in reqs]
+    got = drain(srv)
+    srv.close()
+    assert sorted(got) == sorted(ids)
+    for r, i in zip(reqs, ids):
+        close(got[i], sv.request_plain(r), r.kind, 1e-9)
