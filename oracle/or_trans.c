@@ -61,6 +61,23 @@ void or_sph_bessel(int J, int lmax, double complex xd, double complex *out) {
 		for (int l = 0; l <= lmax; ++l) out[l] = (J == 1) ? (l == 0 ? 1 : 0) : NAN + I * NAN;
 		return;
 	}
+	if (J >= 3 && cimagl(x) != 0) {
+		/* spherical Hankel functions directly, DLMF 10.49.6:
+		 * h^(1,2)_n(z) = (-+i)^{n+1} e^{+-iz}/z sum_k (+-i/(2z))^k (n+k)!/(k!(n-k)!) */
+		const ld sg = (J == 3) ? 1 : -1;
+		const ldc e = cexpl(sg * I * x) / x;
+		for (int n = 0; n <= lmax; ++n) {
+			ldc sum = 0, term = 1;
+			for (int k = 0; k <= n; ++k) {
+				if (k > 0) term *= sg * I / (2 * x) * (ld)(n + k) * (n - k + 1) / k;
+				sum += term;
+			}
+			ldc ph = 1;
+			for (int q = 0; q <= n; ++q) ph *= -sg * I;
+			out[n] = (double complex)(ph * e * sum);
+		}
+		return;
+	}
 	ldc *j = malloc(sizeof(ldc) * (lmax + 2));
 	ldc *y = malloc(sizeof(ldc) * (lmax + 2));
 	ldc s = csinl(x), c = ccosl(x);

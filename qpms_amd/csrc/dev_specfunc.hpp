@@ -215,27 +215,37 @@ QPMS_HD inline void sph_bessel_fill(int kind, cplx x, cplx *out) {
       return;
     }
     sph_jy_real<L>(x.re, j, y);
+    // real part j (y for kind 2), imaginary part +y (h+), -y (h-) or 0
 #pragma unroll
     for (int l = 0; l <= L; ++l) {
-      switch (kind) {
-        case 1: out[l] = mk(j[l], 0.0); break;
-        case 2: out[l] = mk(y[l], 0.0); break;
-        case 3: out[l] = mk(j[l], y[l]); break;
-        default: out[l] = mk(j[l], -y[l]); break;
-      }
+      const double re = (kind == 2) ? y[l] : j[l];
+      const double im = (kind == 3) ? y[l] : ((kind == 4) ? -y[l] : 0.0);
+      out[l] = mk(re, im);
+    }
+  } else if (kind >= 3) {
+    // Hankel functions of complex argument directly (DLMF 10.49.6, 10.49.1):
+    // h0 = -i e^{iz}/z, h1 = -e^{iz}(z + i)/z^2 (and the conjugate forms for
+    // h^(2)), then upward recurrence, which is stable for h. Forming j + i y
+    // would lose everything to cancellation once |Im z| is large.
+    const double sg = (kind == 3) ? 1.0 : -1.0;
+    const cplx e = cexp_(mk(-sg * x.im, sg * x.re));   // e^{+-iz}
+    const cplx zi = cinv(x);
+    cplx h0 = mk(sg * (e * zi).im, -sg * (e * zi).re);  // -+i e^{+-iz}/z
+    cplx h1 = -(e * (x + mk(0.0, sg)) * (zi * zi));
+    out[0] = h0;
+    if (L >= 1) out[1] = h1;
+#pragma unroll
+    for (int l = 1; l < L; ++l) {
+      const cplx h2 = (double)(2 * l + 1) * (zi * h1) - h0;
+      out[l + 1] = h2;
+      h0 = h1;
+      h1 = h2;
     }
   } else {
     cplx j[L + 1], y[L + 1];
     sph_jy_cplx<L>(x, j, y);
 #pragma unroll
-    for (int l = 0; l <= L; ++l) {
-      switch (kind) {
-        case 1: out[l] = j[l]; break;
-        case 2: out[l] = y[l]; break;
-        case 3: out[l] = j[l] + mk(-y[l].im, y[l].re); break;
-        default: out[l] = j[l] - mk(-y[l].im, y[l].re); break;
-      }
-    }
+    for (int l = 0; l <= L; ++l) out[l] = (kind == 1) ? j[l] : y[l];
   }
 }
 

@@ -73,6 +73,21 @@ static const qh_e3dev *qh_e3_get(const qpms_ewald3_constants_t *c) {
 	return e;
 }
 
+/* Drop the device copy of a table that is being freed (its address may be
+ * reused by a later table of another size). */
+void qh_e3_release(const qpms_ewald3_constants_t *c) {
+	pthread_mutex_lock(&qh_e3lock);
+	for (qh_e3dev **pp = &qh_e3reg; *pp; pp = &(*pp)->next)
+		if ((*pp)->c == c) {
+			qh_e3dev *e = *pp;
+			*pp = e->next;
+			qh_dfree(e->leg0); qh_dfree(e->s1); qh_dfree(e->s1off);
+			free(e);
+			break;
+		}
+	pthread_mutex_unlock(&qh_e3lock);
+}
+
 static void qh_check_le(int LE) {
 	QH_ENSURE(LE == 3 || LE == 5 || LE == 7 || LE == 9,
 		"Ewald constants with lMax %d are not instantiated on the GPU (3, 5, 7, 9 = 2*lMax+1 for lMax 1..4)", LE);

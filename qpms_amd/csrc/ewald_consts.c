@@ -7,6 +7,8 @@
  * Condon–Shortley sign.
  */
 #include "qpms_internal.h"
+
+void qh_e3_release(const qpms_ewald3_constants_t *c);
 #include <complex.h>
 #include <math.h>
 
@@ -124,6 +126,7 @@ qpms_ewald3_constants_t *qpms_ewald3_constants_init(qpms_l_t lMax, int csphase) 
 }
 
 void qpms_ewald3_constants_free(qpms_ewald3_constants_t *c) {
+	qh_e3_release(c);
 	if (!c) return;
 	free(c->legendre0); free(c->legendre_plus1); free(c->legendre_minus1);
 	free(c->s1_constfacs); free(c->s1_constfacs_base);

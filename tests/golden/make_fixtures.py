@@ -29,6 +29,20 @@ def sph_bessel_fixtures():
     return dict(bessel_args=args, bessel_j=J, bessel_y=Y)
 
 
+def hankel_fixtures():
+    """h^(1)_l(z) at complex z with large |Im z|, where j + i y cancels."""
+    zs = [20 + 8j, 50 + 20j, 5 + 12j, 0.5 + 3j, 100 + 35j, 2 - 6j, 30 - 1j]
+    L = 17
+    H = np.empty((len(zs), L + 1), np.complex128)
+    with mp.workdps(80):
+        for i, z in enumerate(zs):
+            zz = mp.mpc(z.real, z.imag)
+            for l in range(L + 1):
+                pre = mp.sqrt(mp.pi / (2 * zz))
+                H[i, l] = complex(pre * (mp.besselj(l + 0.5, zz) + 1j * mp.bessely(l + 0.5, zz)))
+    return dict(hankel_args=np.array(zs), hankel_h1=H)
+
+
 def legendre_fixtures():
     # exact Ferrers functions with the Condon-Shortley phase at rational x
     xs = [sympy.Rational(-999, 1000), sympy.Rational(-3, 10), sympy.Integer(0), sympy.Rational(1, 5),
@@ -106,6 +120,7 @@ def ewald_fixtures():
 if __name__ == "__main__":
     d = {}
     d.update(sph_bessel_fixtures())
+    d.update(hankel_fixtures())
     d.update(legendre_fixtures())
     d.update(gaunt_fixtures())
     d.update(ewald_fixtures())

@@ -39,6 +39,21 @@ def test_ab_arrays_single(gpu, lMax, J):
         assert relerr(B, Bo) < TOL, (kr, th, ph)
 
 
+@pytest.mark.parametrize("J", [3, 4])
+def test_ab_arrays_hankel_large_imaginary_argument(gpu, J):
+    """Strongly absorbing media: kr with |Im| up to 12, where j +- i y cancels;
+    the GPU forms h directly and must agree with the oracle's direct form."""
+    import qpms_amd
+    c = qpms_amd.TransCalculator(4)
+    o = oracle_ffi.OracleCalc(4)
+    for kr in (20 + 8j, 5 + 12j, 0.5 + 3j, 30 + 1j, 3 + 0.5j):
+        for th, ph in ((0.4, 1.0), (np.pi / 2, -2.0)):
+            A, B = c.get_AB_arrays(kr, th, ph, J=J)
+            Ao, Bo = o.get_AB_arrays(kr, th, ph, J=J)
+            assert relerr(A, Ao) < TOL, (kr, th, ph)
+            assert relerr(B, Bo) < TOL, (kr, th, ph)
+
+
 def test_ab_arrays_r0_nan(gpu):
     import qpms_amd
     c = qpms_amd.TransCalculator(2)

@@ -31,6 +31,16 @@ def test_bessel_vs_mpmath():
         assert np.max(np.abs(h - (J[i] + 1j * Y[i])) / np.abs(J[i] + 1j * Y[i])) < 1e-13, z
 
 
+def test_hankel_large_imaginary_part_vs_mpmath():
+    """h^(1) of complex argument where forming j + i y would cancel
+    (|Im z| up to 35): the direct form must hold to 1e-13."""
+    for z, H in zip(GOLD["hankel_args"], GOLD["hankel_h1"]):
+        h = of.sph_bessel(3, 17, z)
+        assert np.max(np.abs(h - H) / np.abs(H)) < 1e-13, z
+        hm = of.sph_bessel(4, 17, np.conj(z))   # h^(2)(conj z) = conj h^(1)(z)
+        assert np.max(np.abs(hm - np.conj(H)) / np.abs(H)) < 1e-13, z
+
+
 def test_legendre_vs_sympy():
     for x, P in zip(GOLD["legendre_x"], GOLD["legendre_P"]):
         got = of.legendre_array(9, x, -1)
