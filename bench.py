@@ -229,8 +229,118 @@ def cpu_baseline_scatter(reqs, budget_s, threads):
     return done, t_used
 
 
+def bench_ewald(L, ev, stream):
+    """2D-periodic Ewald sums: hexagonal lattice a = 375 nm, shifts {0, +s, -s}
+    (s = a/sqrt3), lMax 3 (order 7), eta = 2.5/a, R: 12 rings, K: |K| < 30 2pi/a;
+    8192 (beta, omega) points (64 x 64 over the first zone x 2 frequencies in
+    1.5-2.3 eV, 5 % moved to within 1e-4 of the light cone) x 3 shifts, FP64."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ewald_common as ec
+    import qpms_amd
+    rng = np.random.default_rng(20261202)
+    a = ec.A_LAT
+    s = a / np.sqrt(3)
+    b1 = np.array(ec.RB1)
+    b2 = np.array(ec.RB2)
+    tri = []
+    for eV in (1.5, 2.3):
+        k = ec.wavenumber(eV)
+        for i in range(64):
+            for j in range(64):
+                beta = ((i + 0.5) / 64 - 0.5) * b1 + ((j + 0.5) / 64 - 0.5) * b2
+                if rng.uniform() < 0.05:   # onto the light cone of the nearest order, within 1e-4 k
+                    d = rng.normal(size=2)
+                    beta = d / np.linalg.norm(d) * k * (1 + rng.uniform(-1e-4, 1e-4))
+                for sh in ((0, 0), (s, 0), (-s, 0)):
+                    tri.append([beta[0], beta[1], 0, k, 0, sh[0], sh[1], 0])
+    tri = np.array(tri)
+    R = ec.xyweb(ec.B1, ec.B2, 0, True, 12.5 * ec.H_R, False)
+    R3 = np.concatenate([R, np.zeros((len(R), 1))], axis=1)
+    K = ec.K_points()
+    Le = ec.lib_ewald()
+    e3c = Le.qpms_ewald3_constants_init(7, -1)
+    n = len(tri)
+    dtr, dR, dK = (qpms_amd.DeviceBuffer.from_array(x) for x in (tri, R3, K))
+    nsc = 64
+    dss, dsl = qpms_amd.DeviceBuffer(n * nsc * 16), qpms_amd.DeviceBuffer(n * nsc * 16)
+    times = {}
+    for part, (ps, pl) in (("short", (dss.ptr, None)), ("long", (None, dsl.ptr))):
+        Le.qpms_hip_ewald3_sigma_batch_dev(e3c, ec.ETA, ec.AREA, n, dtr.ptr, len(R3), dR.ptr, len(K), dK.ptr, ps, pl,
+                                           None)
+        L.qpms_hip_sync()
+        a0, a1 = ev.create(), ev.create()
+        ev.record(a0, stream)
+        Le.qpms_hip_ewald3_sigma_batch_dev(e3c, ec.ETA, ec.AREA, n, dtr.ptr, len(R3), dR.ptr, len(K), dK.ptr, ps, pl,
+                                           None)
+        ev.record(a1, stream)
+        times[part] = ev.elapsed_ms(a0, a1)
+    # parity on a sample of triples against the oracle
+    ss = dss.to_array(np.complex128, (n, nsc))
+    sl = dsl.to_array(np.complex128, (n, nsc))
+    err = 0.0
+    Rs = R[1:]
+    for t in range(0, n, n // 12):
+        k, beta, sh = tri[t, 3], tri[t, :2], tri[t, 5:7]
+        rr = Rs if (sh == 0).all() else R
+        o_s = ec.oracle_sigma_short(7, k, beta, sh, rr)
+        o_l = ec.oracle_sigma_long(7, k, beta, sh, K)
+        err = max(err, np.max(np.abs(ss[t] - o_s)) / np.max(np.abs(o_s)),
+                  np.max(np.abs(sl[t] - o_l)) / np.max(np.abs(o_l)))
+    Le.qpms_ewald3_constants_free(e3c)
+    tot = times["short"] + times["long"]
+    return {"metric": "Ewald sigma_short + sigma_long triples/s (FP64)", "value": n / (tot * 1e-3),
+            "unit": "(beta, omega, shift) triples/s", "triples": n, "R_points": len(R), "K_points": len(K),
+            "short_ms": times["short"], "long_ms": times["long"],
+            "short_ns_per_point_pair": times["short"] * 1e6 / (n * len(R)),
+            "long_ns_per_point_pair": times["long"] * 1e6 / (n * len(K)),
+            "parity_max_rel_err_sample": err}
+
+
+def bench_beyn(L, window=None):
+    """Beyn contour on a 24 x 24 cluster (pitch 400 nm, r 40 nm, eps -12+1i,
+    n 1.52, lMax 2: m = 9216), ellipse 2.05 eV +- 0.05 eV (Re) x 0.005 eV
+    (Im), 96 nodes, 64 probe columns: batched FP32 node LUs + FP64
+    refinement in the serving loop, against the FP64 path (beyn_solve_v with
+    M from the public builder at every node)."""
+    from qpms_amd import serve as sv
+    g = (np.arange(24) - 11.5) * 400e-9
+    pos = np.array([(x, y, 0.0) for x in g for y in g])
+    req = sv.Req(sv.KIND_BEYN, pos, 2, 40e-9, -12 + 1j, 1.52 ** 2, 2.05 * EV, beyn_rRe=0.05 * EV,
+                 beyn_rIm=0.005 * EV, beyn_nodes=96, beyn_l=64, beyn_seed=1)
+    out = {}
+    for mixed in (True, False):
+        srv = sv.Server(arena_gib=160, mixed=mixed)
+        t0 = time.perf_counter()
+        rid = srv.submit(req)
+        got = None
+        while srv.pending():
+            srv.progress(wait=True)
+            for r in srv.poll():
+                got = r
+        dt = time.perf_counter() - t0
+        srv.close()
+        out["mixed" if mixed else "fp64"] = (got, dt)
+        log(f"beyn {'mixed' if mixed else 'fp64'}: {dt:.2f} s, neig {got.neig}")
+    t0 = time.perf_counter()
+    plain = sv.request_plain(req)
+    tp = time.perf_counter() - t0
+    log(f"beyn plain FP64 path: {tp:.2f} s")
+    gm, gf = out["mixed"][0], out["fp64"][0]
+    svm, svp = np.array(gm.sv[:64]), np.array(plain.sv[:64])
+    return {"metric": "Beyn contour (m = 9216, 96 nodes, l = 64) wall time", "unit": "s",
+            "mixed_s": out["mixed"][1], "fp64_loop_s": out["fp64"][1], "plain_fp64_s": tp,
+            "speedup_vs_plain": tp / out["mixed"][1],
+            "node_lu_flops": 96 * 8 / 3 * 9216.0 ** 3,
+            "mixed_node_lu_TFLOPs": 96 * 8 / 3 * 9216.0 ** 3 / out["mixed"][1] / 1e12,
+            "neig": [gm.neig, gf.neig, plain.neig],
+            "sv_max_rel_diff_mixed_vs_plain": float(np.max(np.abs(svm - svp)) / svp[0]),
+            "eigval_mixed": [str(z) for z in gm.eigenvalues()], "eigval_plain": [str(z) for z in
+                                                                               plain.eigenvalues()]}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["serving", "ewald", "beyn"], default="serving")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -259,6 +369,12 @@ def main():
 
     if args.lu_probe:
         print(json.dumps(measure_lu(L, ev, stream, reps=2)), flush=True)
+        return
+    if args.workload == "ewald":
+        print(json.dumps(bench_ewald(L, ev, stream)), flush=True)
+        return
+    if args.workload == "beyn":
+        print(json.dumps(bench_beyn(L)), flush=True)
         return
 
     wb = None if args.no_wbuild else measure_w_build(L, ev, stream)

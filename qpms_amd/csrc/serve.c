@@ -331,7 +331,7 @@ unsigned long long qpms_hip_server_submit(qpms_hip_server_t *s, const qpms_hip_r
 	j->L = job_L(r);
 	j->nb = 2 * (size_t)qpms_lMax2nelem(r->lMax);
 	j->t_submit = qpms_hip_server_clock(s);
-	if (job_bytes(s, j, NULL) > s->arena.size) { free(j->pos); free(j); return 0; }
+	if (((job_bytes(s, j, NULL) + 4095) & ~(size_t)4095) > s->arena.size) { free(j->pos); free(j); return 0; }
 	j->id = s->next_id++;
 	if (s->qtail) s->qtail->next = j; else s->qhead = j;
 	s->qtail = j;
