@@ -197,14 +197,13 @@ def replay(stream_reqs, window, mixed=True, arena_gib=160, drain_cancel=True):
     return got, nxt, wall
 
 
-def rel_diff(a, b, kind):
+def rel_diff(a, b, req):
     from qpms_amd import serve as sv
-    if kind == sv.KIND_SCATTER:
+    if req.kind == sv.KIND_SCATTER:
         return abs(a.sigma_ext - b.sigma_ext) / abs(b.sigma_ext)
-    if kind == sv.KIND_PERIODIC:
+    if req.kind == sv.KIND_PERIODIC:
         return abs(a.logabsdet - b.logabsdet) / max(1.0, abs(b.logabsdet))
-    sa, sb = np.array(a.sv[:32]), np.array(b.sv[:32])
-    return float(np.max(np.abs(sa - sb)) / sb[0]) if sb[0] > 0 else 0.0
+    return sv.beyn_rel_diff(a, b, req.c.rank_tol)
 
 
 def cpu_baseline_scatter(reqs, budget_s, threads):
@@ -306,7 +305,8 @@ def bench_beyn(L, window=None):
     g = (np.arange(24) - 11.5) * 400e-9
     pos = np.array([(x, y, 0.0) for x in g for y in g])
     req = sv.Req(sv.KIND_BEYN, pos, 2, 40e-9, -12 + 1j, 1.52 ** 2, 2.05 * EV, beyn_rRe=0.05 * EV,
-                 beyn_rIm=0.005 * EV, beyn_nodes=96, beyn_l=64, beyn_seed=1)
+                 beyn_rIm=0.005 * EV, beyn_nodes=96, beyn_l=64, beyn_seed=1,
+                 rank_tol=sv.beyn_rank_tol(9216, 64, 0.05 * EV))
     out = {}
     for mixed in (True, False):
         srv = sv.Server(arena_gib=160, mixed=mixed)
@@ -333,7 +333,9 @@ def bench_beyn(L, window=None):
             "node_lu_flops": 96 * 8 / 3 * 9216.0 ** 3,
             "mixed_node_lu_TFLOPs": 96 * 8 / 3 * 9216.0 ** 3 / out["mixed"][1] / 1e12,
             "neig": [gm.neig, gf.neig, plain.neig],
-            "sv_max_rel_diff_mixed_vs_plain": float(np.max(np.abs(svm - svp)) / svp[0]),
+            "rank_tol": req.c.rank_tol, "sv_top": [float(x) for x in svp[:8]],
+            "max_rel_diff_mixed_vs_plain": sv.beyn_rel_diff(gm, plain, req.c.rank_tol),
+            "max_rel_diff_fp64_loop_vs_plain": sv.beyn_rel_diff(gf, plain, req.c.rank_tol),
             "eigval_mixed": [str(z) for z in gm.eigenvalues()], "eigval_plain": [str(z) for z in
                                                                                plain.eigenvalues()]}
 
@@ -418,7 +420,7 @@ def main():
             t0 = time.perf_counter()
             pr = sv.request_plain(reqs[i])
             times.append(time.perf_counter() - t0)
-            errs.append(rel_diff(ok[i][0], pr, reqs[i].kind))
+            errs.append(rel_diff(ok[i][0], pr, reqs[i]))
             kinds_p.append(reqs[i].kind)
             if time.perf_counter() - t_start > args.plain_budget:
                 break

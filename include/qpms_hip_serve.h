@@ -45,7 +45,7 @@ typedef struct qpms_hip_request_t {
 	double radius;
 	_Complex double eps_in;
 	double eps_bg;
-	double omega;               /* rad/s; BEYN: centre of the contour (real part) */
+	double omega;               /* rad/s; BEYN: real part of the contour centre */
 	/* SCATTER: plane wave e^{i k.r} E with k = k_bg * kdir (unit vector) */
 	double kdir[3];
 	_Complex double E[3];
@@ -58,7 +58,8 @@ typedef struct qpms_hip_request_t {
 	int beyn_nodes;
 	int beyn_l;
 	unsigned long long beyn_seed;
-	double rank_tol;
+	double rank_tol;            /* BEYN: absolute singular-value cut (beyn.c:382) */
+	double beyn_centre_im;      /* BEYN: imaginary part of the contour centre (rad/s) */
 } qpms_hip_request_t;
 
 typedef struct qpms_hip_response_t {

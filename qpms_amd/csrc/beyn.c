@@ -25,10 +25,12 @@
 
 int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
 		double *A0, double *A1, double *A0c, double *A1c, void *stream);
-int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, double *sigma_host, double *Vt_out, void *stream);
+int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, double *sigma_host, double *Vt_out, void *stream,
+		void *ws);
+size_t qh_jacobi_svd_ws_bytes(int m, int l);
 int qh_ctrans_gemm(int m, int l, int K, const double *Ut, const double *A, double *C, void *stream);
 int qh_v0s(int m, int K, const double *S, const double *Ut, double *Y, void *stream);
-int qh_small_eig(int K, const double *B, double *lam, double *S, void *stream);
+int qh_small_eig(int K, const double *B, double *lam, double *S, void *stream, void *ws);
 int qh_bmat(int K, int l, const double *T, const double *Vt, const double *sigma, double *B, void *stream);
 
 /* (Re(z - c)/rRe)^2 + (Im(z - c)/rIm)^2 < 1 with the half-axes stored after
@@ -195,32 +197,38 @@ static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params
 		const complex double *dA0, const complex double *dA1, complex double z0,
 		double rank_tol, size_t rank_sel_min, double res_tol,
 		complex double *eigval, complex double *eigvec /* l x m, nullable */, double *residuals, double *sigma,
-		void *stream) {
-	complex double *dUt = qh_dmalloc(m * l * sizeof(complex double));
-	complex double *dVt = qh_dmalloc(l * l * sizeof(complex double));
-	qh_jacobi_svd((int)m, (int)l, (const double *)dA0, (double *)dUt, sigma, (double *)dVt, stream);
+		void *stream, void *ws) {
+	/* device scratch: from `ws` (qh_beyn_ws_bytes) when given, else allocated */
+	char *wp = ws;
+	#define QH_TAKE(b) (ws ? (void *)((wp += (((b) + 255) & ~(size_t)255)) - (((b) + 255) & ~(size_t)255)) : qh_dmalloc(b))
+	#define QH_GIVE(p) do { if (!ws) qh_dfree(p); } while (0)
+	complex double *dUt = QH_TAKE(m * l * sizeof(complex double));
+	complex double *dVt = QH_TAKE(l * l * sizeof(complex double));
+	void *svd_ws = ws ? QH_TAKE(qh_jacobi_svd_ws_bytes((int)m, (int)l)) : NULL;
+	qh_jacobi_svd((int)m, (int)l, (const double *)dA0, (double *)dUt, sigma, (double *)dVt, stream, svd_ws);
 	size_t K = 0;
 	for (size_t k = 0; k < l; ++k)
 		if (k < rank_sel_min || sigma[k] > rank_tol) ++K;
-	if (K == 0) { qh_dfree(dUt); qh_dfree(dVt); return 0; }
+	if (K == 0) { QH_GIVE(dUt); QH_GIVE(dVt); return 0; }
 	QH_ENSURE(K <= 64, "Beyn: %zu candidate eigenvalues exceed the reduced-problem limit of 64", K);
 	/* TM2 = V0^H A1 (K x l) */
-	complex double *dT = qh_dmalloc(K * l * sizeof(complex double));
+	complex double *dT = QH_TAKE(K * l * sizeof(complex double));
 	qh_ctrans_gemm((int)m, (int)l, (int)K, (const double *)dUt, (const double *)dA1, (double *)dT, stream);
 	/* B = TM2 W0 Sigma^{-1}, W0 column j = row j of Vt */
-	double *dsig = qh_dmalloc(l * sizeof(double));
+	double *dsig = QH_TAKE(l * sizeof(double));
 	qh_h2d(dsig, sigma, l * sizeof(double), stream);
-	complex double *dB = qh_dmalloc((K * K + K + K * K) * sizeof(complex double));
+	complex double *dB = QH_TAKE((K * K + K + K * K) * sizeof(complex double));
 	complex double *dlam = dB + K * K, *dS = dlam + K;
 	qh_bmat((int)K, (int)l, (const double *)dT, (const double *)dVt, dsig, (double *)dB, stream);
-	int fail = qh_small_eig((int)K, (const double *)dB, (double *)dlam, (double *)dS, stream);
+	void *eig_ws = ws ? QH_TAKE(256) : NULL;
+	int fail = qh_small_eig((int)K, (const double *)dB, (double *)dlam, (double *)dS, stream, eig_ws);
 	QH_ENSURE(!fail, "Beyn: the reduced eigenproblem did not converge");
 	complex double *lam;
 	QH_MALLOC(lam, K * sizeof(complex double));
 	qh_d2h(lam, dlam, K * sizeof(complex double), stream);
 	complex double *dY = NULL, *Y = NULL;
 	if (eigvec || res_tol > 0) {
-		dY = qh_dmalloc(K * m * sizeof(complex double));
+		dY = QH_TAKE(K * m * sizeof(complex double));
 		qh_v0s((int)m, (int)K, (const double *)dS, (const double *)dUt, (double *)dY, stream);
 		QH_MALLOC(Y, K * m * sizeof(complex double));
 		qh_d2h(Y, dY, K * m * sizeof(complex double), stream);
@@ -253,17 +261,31 @@ static int qh_beyn_process(size_t m, size_t l, beyn_function_M_t M, void *params
 		}
 		++kept;
 	}
-	qh_dfree(dMm); qh_dfree(dr); qh_dfree(dsig);
+	qh_dfree(dMm); qh_dfree(dr);
+	QH_GIVE(dsig);
 	free(Mm); free(Y); free(lam);
-	qh_dfree(dY); qh_dfree(dB); qh_dfree(dT); qh_dfree(dUt); qh_dfree(dVt);
+	if (dY) QH_GIVE(dY);
+	QH_GIVE(dB); QH_GIVE(dT); QH_GIVE(dUt); QH_GIVE(dVt);
+	#undef QH_TAKE
+	#undef QH_GIVE
 	return (int)kept;
+}
+
+/* Device scratch of the tail without residuals, for callers that cannot
+ * allocate while other work runs (the serving loop). */
+size_t qh_beyn_ws_bytes(size_t m, size_t l) {
+	const size_t a = 255;
+	#define R256(b) ((((size_t)(b)) + a) & ~a)
+	return R256(m * l * 16) + R256(l * l * 16) + R256(qh_jacobi_svd_ws_bytes((int)m, (int)l)) + R256(l * l * 16)
+		+ R256(l * 8) + R256((2 * l * l + l) * 16) + 256 + R256(l * m * 16) + 1024;
+	#undef R256
 }
 
 /* Serving-loop entry: eigenvalues only, no residual filter. */
 int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
-		double rank_tol, size_t rank_sel_min, complex double *eigval, double *sigma, void *stream) {
+		double rank_tol, size_t rank_sel_min, complex double *eigval, double *sigma, void *stream, void *ws) {
 	return qh_beyn_process(m, l, NULL, NULL, dA0, dA1, z0, rank_tol, rank_sel_min, 0.0, eigval, NULL, NULL, sigma,
-		stream);
+		stream, ws);
 }
 
 beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_function_M_inv_Vhat_t M_inv_Vhat,
@@ -327,9 +349,9 @@ beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_functi
 	QH_MALLOC(sv_coarse, (l ? l : 1) * sizeof(double));
 	/* coarse pass first (its values become the error estimate), then fine */
 	qh_beyn_process(m, l, M, params, dA0c, dA1c, z0, rank_tol, rank_sel_min, res_tol,
-		r->eigval_err, NULL, NULL, sv_coarse, NULL);
+		r->eigval_err, NULL, NULL, sv_coarse, NULL, NULL);
 	const int K = qh_beyn_process(m, l, M, params, dA0, dA1, z0, rank_tol, rank_sel_min, res_tol,
-		r->eigval, r->eigvec, r->residuals, r->ranktest_SV, NULL);
+		r->eigval, r->eigvec, r->residuals, r->ranktest_SV, NULL, NULL);
 	for (size_t i = 0; i < l; ++i) r->eigval_err[i] -= r->eigval[i];   /* beyn.c:592 */
 	r->neig = (size_t)K;
 	free(sv_coarse);

@@ -352,13 +352,28 @@ extern "C" int qh_beyn_accum(long n, const double *X, double dzr, double dzi, do
 // One-sided Jacobi SVD of A (m x l, row-major, device). Outputs: Ut (l x m)
 // left singular vectors as rows (sorted by decreasing sigma), sigma (l, host),
 // Vt (l x l) rows = right singular vectors (column j of V as row j).
+// Device scratch of qh_jacobi_svd (bytes), carved from `ws` when given.
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+extern "C" size_t qh_jacobi_svd_ws_bytes(int m, int l) {
+  const int n = (l % 2) ? l + 1 : l;
+  return a256((size_t)m * l * sizeof(v2d)) + a256((size_t)l * l * sizeof(v2d)) + 256 + a256((size_t)l * 8) +
+         a256((size_t)(n - 1) * (n / 2) * 2 * sizeof(int));
+}
+
 extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, double *sigma_host, double *Vt_out,
-                             void *stream) {
+                             void *stream, void *ws) {
   hipStream_t s = (hipStream_t)qh_stream(stream);
-  v2d *Ut = (v2d *)qh_dmalloc((size_t)m * l * sizeof(v2d));
-  v2d *Vt = (v2d *)qh_dmalloc((size_t)l * l * sizeof(v2d));
-  unsigned long long *doff = (unsigned long long *)qh_dmalloc(sizeof(unsigned long long));
-  double *dn = (double *)qh_dmalloc((size_t)l * sizeof(double));
+  char *w = (char *)ws;
+  auto take = [&](size_t b) -> void * {
+    if (!ws) return qh_dmalloc(b);
+    void *r = w;
+    w += a256(b);
+    return r;
+  };
+  v2d *Ut = (v2d *)take((size_t)m * l * sizeof(v2d));
+  v2d *Vt = (v2d *)take((size_t)l * l * sizeof(v2d));
+  unsigned long long *doff = (unsigned long long *)take(256);
+  double *dn = (double *)take((size_t)l * sizeof(double));
   hipLaunchKernelGGL(k_transpose, dim3((unsigned)(((long)m * l + 255) / 256)), dim3(256), 0, s, m, l, (const v2d *)A, Ut);
   v2d *I = (v2d *)calloc((size_t)l * l, sizeof(v2d));
   for (int i = 0; i < l; ++i) I[i * l + i] = v2d{1.0, 0.0};
@@ -380,7 +395,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
     for (int i = n - 1; i > 1; --i) order[i] = order[i - 1];
     order[1] = last;
   }
-  int *dpr = (int *)qh_dmalloc((size_t)(n - 1) * npr * 2 * sizeof(int));
+  int *dpr = (int *)take((size_t)(n - 1) * npr * 2 * sizeof(int));
   HIPCHK(hipMemcpyAsync(dpr, pr, (size_t)(n - 1) * npr * 2 * sizeof(int), hipMemcpyHostToDevice, s));
   for (int sweep = 0; sweep < 40; ++sweep) {
     HIPCHK(hipMemsetAsync(doff, 0, sizeof(unsigned long long), s));
@@ -393,7 +408,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
     memcpy(&offd, &off, sizeof offd);
     if (offd < 1e-15) break;
   }
-  qh_dfree(dpr);
+  if (!ws) qh_dfree(dpr);
   hipLaunchKernelGGL(k_row_norms, dim3(l), dim3(256), 0, s, m, (const v2d *)Ut, dn);
   double *nrm = (double *)malloc(l * sizeof(double));
   HIPCHK(hipMemcpyAsync(nrm, dn, l * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -424,7 +439,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
   HIPCHK(hipMemcpyAsync(Vt_out, oV, (size_t)l * l * sizeof(v2d), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   free(I); free(order); free(pr); free(nrm); free(idx); free(hU); free(hV); free(oU); free(oV);
-  qh_dfree(Ut); qh_dfree(Vt); qh_dfree(doff); qh_dfree(dn);
+  if (!ws) { qh_dfree(Ut); qh_dfree(Vt); qh_dfree(doff); qh_dfree(dn); }
   return 0;
 }
 
@@ -452,16 +467,16 @@ extern "C" int qh_bmat(int K, int l, const double *T, const double *Vt, const do
   return 0;
 }
 
-extern "C" int qh_small_eig(int K, const double *B, double *lam, double *S, void *stream) {
+extern "C" int qh_small_eig(int K, const double *B, double *lam, double *S, void *stream, void *ws) {
   if (K <= 0) return 0;
   if (K > KMAX) qpms_hip_die(__FILE__, __LINE__, __func__, "reduced eigenproblem of order %d exceeds %d", K, KMAX);
   hipStream_t s = (hipStream_t)qh_stream(stream);
-  int *st = (int *)qh_dmalloc(sizeof(int));
+  int *st = ws ? (int *)ws : (int *)qh_dmalloc(sizeof(int));
   hipLaunchKernelGGL(k_small_eig, dim3(1), dim3(64), 0, s, K, (const v2d *)B, (v2d *)lam, (v2d *)S, st);
   HIPCHK(hipGetLastError());
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, st, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  qh_dfree(st);
+  if (!ws) qh_dfree(st);
   return h;
 }

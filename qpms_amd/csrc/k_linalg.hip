@@ -7,12 +7,12 @@
 // beyn.c:556-561 (zgetrf + zgetrs with l right-hand sides).
 //
 // LU structure (getrf_batched): outer blocks of LU_NB = 256 columns.
-//   k_tslu_*      factor a tall strip of PW columns (64 in FP32, 32 in
-//                 FP64) with tournament pivoting: candidate rows chosen per
-//                 chunk in LDS, merged pairwise, the winners moved to the top
-//                 (recorded as zgetrf-style interchanges), then L21 by a
+//   k_ts_*        factor a tall strip of PW = 32 columns with tournament
+//                 pivoting: candidate rows chosen per 512-row chunk (one row
+//                 per thread, in registers), merged 16 sets at a time, the
+//                 winners moved to the top (recorded as zgetrf-style
+//                 interchanges) in every column by one gather, then L21 by a
 //                 triangular solve; no workgroup waits for another;
-//   k_laswp       applies the strip's row swaps to every other column;
 //   k_trsm        U = L^{-1} B for a PW-row block (one thread per column,
 //                 the unit-lower triangle in LDS);
 //   k_gemm_sub    C -= A B: the rank-PW updates inside an outer block and
@@ -58,27 +58,6 @@ template <class T> __device__ __forceinline__ typename C2<T>::t cdiv(typename C2
   } else {
     T r = b.x / b.y, d = b.x * r + b.y;
     return typename C2<T>::t{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
-  }
-}
-
-// Row swaps of rows k0..k0+w-1 (pivots ipiv[k0..]) on columns outside the panel.
-template <class T>
-__global__ __launch_bounds__(256) void k_laswp(int n, int k0, int w, typename C2<T>::t *A, long lda, long sA,
-                                               const int *ipiv, long sP) {
-  typedef typename C2<T>::t cT;
-  A += (long)blockIdx.z * sA;
-  ipiv += (long)blockIdx.z * sP;
-  const int c0 = blockIdx.x * 256 + threadIdx.x;
-  const int ncols = n - w;  // columns outside the panel
-  if (c0 >= ncols) return;
-  const int c = (c0 < k0) ? c0 : c0 + w;
-  for (int j = 0; j < w; ++j) {
-    const int r = k0 + j, p = ipiv[r] - 1;
-    if (p != r) {
-      cT a = A[(long)r * lda + c], b = A[(long)p * lda + c];
-      A[(long)r * lda + c] = b;
-      A[(long)p * lda + c] = a;
-    }
   }
 }
 
@@ -315,262 +294,259 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 // ---------------------------------------------------------------------------
 // Tall-strip panel: LU with tournament pivoting (CALU / TSLU, Grigori,
 // Demmel & Xiang 2011). The strip (rows c0..n-1, columns c0..c0+w-1,
-// w <= PW) is split into chunks of RPB rows:
-//   k_tslu_local  each workgroup runs partial-pivoting elimination on its
-//                 chunk in LDS only to choose w candidate rows, and writes
-//                 their indices and original values;
-//   k_tslu_merge  pairs of candidate sets are merged the same way, level by
-//                 level, until one set of w rows remains; the order in which
-//                 the last merge chose them is the pivot order;
-//   k_tslu_top    one workgroup per matrix moves those rows to the top as w
-//                 row interchanges (ipiv, perm), factors the w x w block
-//                 without further pivoting and sets info on a zero pivot;
-//   k_tslu_l21    every other strip row gets L21 = A21 U11^{-1}.
+// w <= PW) is split into chunks of TS_NT rows, one row per thread held in
+// registers:
+//   k_ts_local   each workgroup runs partial-pivoting elimination on its
+//                chunk only to choose w candidate rows, and writes their
+//                indices and original values;
+//   k_ts_merge   up to TS_FAN candidate sets per workgroup are merged the
+//                same way, level by level, until one set remains;
+//   the last level (local when one chunk covers the strip, else the last
+//                merge) also writes the w x w block L11\U11 as its
+//                elimination left the pivot rows, the zgetrf-style
+//                interchanges (ipiv, perm), info on a zero pivot, and the
+//                list of rows the interchanges move;
+//   k_ts_apply   moves those rows in every column of the matrix (one gather
+//                per column block) and stores L11\U11;
+//   k_tslu_l21   every other strip row gets L21 = A21 U11^{-1}.
 // No workgroup ever waits for another one; each step is its own launch.
 // ---------------------------------------------------------------------------
-constexpr int PANEL_LDS = 64 * 1024;
+constexpr int PW_ = 32;           // strip width (FP32 and FP64)
+constexpr int TS_NT = 512;        // rows per tournament workgroup
+constexpr int TS_WAVES = TS_NT / 64;
+constexpr int TS_FAN = TS_NT / PW_;   // candidate sets merged per workgroup
 
-template <class T> struct PanelW;
-template <> struct PanelW<float> { static constexpr int W = 32; };
-template <> struct PanelW<double> { static constexpr int W = 32; };
+template <class T> struct PanelW { static constexpr int W = PW_; };
 
-template <class T>
-static int panel_rows_per_block() { return PANEL_LDS / (PanelW<T>::W * (int)sizeof(typename C2<T>::t)); }
-
-// Per matrix: two candidate buffers of Gmax sets x PW rows (index + PW values).
+// Per matrix: two candidate buffers of Gmax sets (index + PW values each),
+// then the last level's output: L11\U11, the moved-row list and its length.
 template <class T>
 struct PanelWS {
-  static size_t set_bytes() {
-    constexpr int PW = PanelW<T>::W;
-    return (size_t)PW * 4 + (size_t)PW * PW * sizeof(typename C2<T>::t);
-  }
-  static size_t bytes(int G) { return 2 * (size_t)G * ((set_bytes() + 255) / 256 * 256) + 256; }
+  static size_t set_bytes() { return ((size_t)PW_ * 4 + (size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 255) / 256 * 256; }
+  static size_t fin_bytes() { return ((size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 4 * PW_ * 4 + 16 + 255) / 256 * 256; }
+  static int sets(int n) { return (n + TS_NT - 1) / TS_NT; }
+  static size_t bytes(int n) { return (2 * (size_t)sets(n) * set_bytes() + fin_bytes() + 255) / 256 * 256; }
 };
 
-// Partial-pivoting selection on X[nrows][PW] (LDS, eliminated in place):
-// the first min(w, nrows) pivot rows in order into sel[]. key[r] orders ties
-// (smaller key wins, as izamax's first maximum). Per column: a wave-shuffle
-// arg-max, one LDS combine of the four waves, the scaling of the column and
-// the rank-1 update of the unused rows (three barriers).
 template <class T>
-__device__ int gepp_select(typename C2<T>::t *X, int nrows, int w, const int *key, int *sel, int *used,
-                           T (*wv)[4], int (*wr)[4]) {
+struct TsShared {
+  T wv[2][TS_WAVES];
+  int wk[2][TS_WAVES], wt[2][TS_WAVES];
+  typename C2<T>::t prow[2][PW_];
+  int skey[PW_];   // key (strip row) of pivot j
+  T sval[PW_];     // |U_jj|_1 at selection
+};
+
+// Partial-pivoting elimination over the rows of a workgroup, row t in the
+// registers x[] of thread t. Per column: a wave-shuffle arg-max of |x_j|_1
+// (ties: smaller key, as izamax's first maximum), one LDS combine of the
+// waves, the pivot row broadcast through LDS, and the rank-1 update of the
+// thread's own row (two barriers). Returns the number of pivots chosen; on
+// return myj is the pivot order of this thread's row (-1 if not chosen) and
+// its x[] holds that row of L\U.
+template <class T>
+__device__ __forceinline__ int ts_gepp(typename C2<T>::t (&x)[PW_], bool act, int key, int w, TsShared<T> &sh, int &myj) {
   typedef typename C2<T>::t cT;
-  constexpr int PW = PanelW<T>::W;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int r = t; r < nrows; r += 256) used[r] = 0;
-  __syncthreads();
-  const int ns = min(w, nrows);
-  for (int j = 0; j < ns; ++j) {
+  bool live = act, done = false;
+  myj = -1;
+  int ns = 0;
+#pragma clang loop unroll(full)
+  for (int j = 0; j < PW_; ++j) {
+    if (done || j >= w) continue;   // uniform
     const int par = j & 1;
-    T best = -1;
-    int bk = 0x7fffffff, br = -1;
-    for (int r = t; r < nrows; r += 256) {
-      if (used[r]) continue;
-      const cT v = X[r * PW + j];
-      const T a = fabs(v.x) + fabs(v.y);
-      if (a > best || (a == best && key[r] < bk)) { best = a; bk = key[r]; br = r; }
+    T a = -1;
+    if (live) {
+      a = fabs(x[j].x) + fabs(x[j].y);
+      if (!(a == a)) a = 0;   // NaN: chosen only when nothing else is left
     }
+    int k = live ? key : 0x7fffffff, tid = t;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      const T v2 = __shfl_xor(best, off);
-      const int k2 = __shfl_xor(bk, off), r2 = __shfl_xor(br, off);
-      if (v2 > best || (v2 == best && k2 < bk)) { best = v2; bk = k2; br = r2; }
+      const T a2 = __shfl_xor(a, off);
+      const int k2 = __shfl_xor(k, off), t2 = __shfl_xor(tid, off);
+      if (a2 > a || (a2 == a && k2 < k)) { a = a2; k = k2; tid = t2; }
     }
-    if (lane == 0) { wv[par][wave] = best; wr[par][wave] = br; }
+    if (lane == 0) { sh.wv[par][wave] = a; sh.wk[par][wave] = k; sh.wt[par][wave] = tid; }
     __syncthreads();
-    T pv = -1;
-    int p = -1, pk = 0x7fffffff;
+    T pa = -1;
+    int pk = 0x7fffffff, pt = -1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const T v = wv[par][q];
-      const int r = wr[par][q];
-      const int k = (r >= 0) ? key[r] : 0x7fffffff;
-      if (v > pv || (v == pv && k < pk)) { pv = v; p = r; pk = k; }
+    for (int q = 0; q < TS_WAVES; ++q) {
+      const T v = sh.wv[par][q];
+      const int kq = sh.wk[par][q];
+      if (v > pa || (v == pa && kq < pk)) { pa = v; pk = kq; pt = sh.wt[par][q]; }
     }
-    if (p < 0) {   // only NaNs left: the first unused row
-      p = 0;
-      while (p < nrows - 1 && used[p]) ++p;
+    if (pa < 0) { done = true; continue; }   // no row left (uniform)
+    if (t == pt) {
+#pragma unroll
+      for (int c = 0; c < PW_; ++c) sh.prow[par][c] = x[c];
+      live = false;
+      myj = j;
     }
-    if (pv > 0) {
-      const cT d = X[p * PW + j];
-      for (int r = t; r < nrows; r += 256)
-        if (!used[r] && r != p) X[r * PW + j] = cdiv<T>(X[r * PW + j], d);
-    }
+    if (t == 0) { sh.skey[j] = pk; sh.sval[j] = pa; }
     __syncthreads();
-    if (t == 0) { sel[j] = p; used[p] = 1; }
-    if (pv > 0) {
-      const int nc = w - j - 1;
-      for (int e = t; e < nrows * nc; e += 256) {
-        const int r = e / nc, c = j + 1 + e % nc;
-        if (!used[r] && r != p) {
-          const cT l = X[r * PW + j], u = X[p * PW + c];
-          cT &x = X[r * PW + c];
-          x = cT{x.x - (l.x * u.x - l.y * u.y), x.y - (l.x * u.y + l.y * u.x)};
-        }
+    ns = j + 1;
+    if (live && pa > 0) {
+      const cT l = cdiv<T>(x[j], sh.prow[par][j]);
+      x[j] = l;
+#pragma unroll
+      for (int c = j + 1; c < PW_; ++c) {
+        const cT u = sh.prow[par][c];
+        x[c] = cT{x[c].x - (l.x * u.x - l.y * u.y), x[c].y - (l.x * u.y + l.y * u.x)};
       }
     }
-    __syncthreads();
   }
   return ns;
 }
 
+// Output of the last tournament level: L11\U11 rows, ipiv (1-based, matrix
+// rows), perm, info, and the compacted list of (from, to) strip rows the
+// interchanges move. The list starts as the w top rows; each interchange
+// i <-> p is applied to it with wave ballots (a row not yet listed sits at
+// its own index).
 template <class T>
-__global__ __launch_bounds__(256) void k_tslu_local(int n, int c0, int w, int RPB, const typename C2<T>::t *A, long lda,
-                                                    long sA, char *ws, long sW, long setb) {
+__device__ __forceinline__ void ts_finish(const typename C2<T>::t (&x)[PW_], int myj, int ns, int w, int c0, TsShared<T> &sh,
+                          char *fin, int *ipiv, int *perm, int *info) {
   typedef typename C2<T>::t cT;
-  constexpr int PW = PanelW<T>::W;
-  extern __shared__ char smem_raw[];
-  cT *X = reinterpret_cast<cT *>(smem_raw);
-  __shared__ int key[512], used[512], sel[PW], wr[2][4];
-  __shared__ T wv[2][4];
+  const int t = threadIdx.x;
+  cT *lu = reinterpret_cast<cT *>(fin);
+  int *lfrom = reinterpret_cast<int *>(fin + PW_ * PW_ * sizeof(cT));
+  int *lto = lfrom + 2 * PW_;
+  int *lcount = lto + 2 * PW_;
+  if (myj >= 0) {
+#pragma unroll
+    for (int c = 0; c < PW_; ++c) lu[myj * PW_ + c] = (c < w) ? x[c] : cT{0, 0};
+  }
+  if (t < 64) {
+    int id = (t < w) ? t : -1, pos = id, nlist = w;
+    for (int i = 0; i < ns; ++i) {
+      const int tg = sh.skey[i];
+      const unsigned long long mt = __ballot(id == tg);
+      const int lt = mt ? __ffsll((long long)mt) - 1 : -1;
+      const int p = (lt >= 0) ? __shfl(pos, lt) : tg;
+      const unsigned long long mi = __ballot(id >= 0 && pos == i);
+      const int li = __ffsll((long long)mi) - 1;
+      if (p != i) {
+        if (t == li) pos = p;
+        if (lt >= 0) {
+          if (t == lt) pos = i;
+        } else {
+          if (t == nlist) { id = tg; pos = i; }
+          ++nlist;
+        }
+      }
+      if (t == 0) {
+        ipiv[c0 + i] = c0 + p + 1;
+        if (sh.sval[i] == 0 && *info == 0) *info = c0 + i + 1;
+      }
+    }
+    const bool mv = (t < nlist) && id != pos;
+    const unsigned long long mm = __ballot(mv);
+    const int slot = __popcll(mm & ((1ull << t) - 1ull));
+    if (mv) { lfrom[slot] = id; lto[slot] = pos; }
+    if (t == 0) *lcount = __popcll(mm);
+    if (perm) {   // the moved rows as one gather: every lane loads before any stores
+      const int v = mv ? perm[c0 + id] : 0;
+      if (mv) perm[c0 + pos] = v;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(TS_NT) void k_ts_local(int n, int c0, int w, const typename C2<T>::t *A, long lda,
+                                                    long sA, char *ws, long sW, long setb, long finoff, int final_,
+                                                    int *ipiv, long sP, int *perm, int *info) {
+  typedef typename C2<T>::t cT;
+  __shared__ TsShared<T> sh;
   const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x;
   A += (long)z * sA;
-  char *set = ws + (long)z * sW + (long)g * setb;
+  ws += (long)z * sW;
+  const int m = n - c0, r = g * TS_NT + t;
+  const bool act = r < m;
+  cT x[PW_];
+  const cT *row = A + (long)(c0 + (act ? r : 0)) * lda + c0;
+#pragma unroll
+  for (int c = 0; c < PW_; ++c) x[c] = (act && c < w) ? row[c] : cT{0, 0};
+  int myj;
+  const int ns = ts_gepp<T>(x, act, r, w, sh, myj);
+  if (final_) {
+    ts_finish<T>(x, myj, ns, w, c0, sh, ws + finoff, ipiv + (long)z * sP, perm ? perm + (long)z * sP : nullptr,
+                 info + z);
+    return;
+  }
+  char *set = ws + (long)g * setb;
   int *oidx = reinterpret_cast<int *>(set);
-  cT *oval = reinterpret_cast<cT *>(set + PW * 4);
-  const int m = n - c0, r0 = g * RPB, nr = max(0, min(RPB, m - r0));
-  for (int e = t; e < nr * PW; e += 256) {
-    const int r = e / PW, c = e % PW;
-    X[e] = (c < w) ? A[(long)(c0 + r0 + r) * lda + c0 + c] : cT{0, 0};
-  }
-  for (int r = t; r < nr; r += 256) key[r] = r0 + r;
-  __syncthreads();
-  const int ns = gepp_select<T>(X, nr, w, key, sel, used, wv, wr);
-  for (int i = t; i < PW; i += 256) oidx[i] = (i < ns) ? r0 + sel[i] : -1;
-  for (int e = t; e < ns * PW; e += 256) {
-    const int i = e / PW, c = e % PW;
-    oval[e] = (c < w) ? A[(long)(c0 + r0 + sel[i]) * lda + c0 + c] : cT{0, 0};
+  cT *oval = reinterpret_cast<cT *>(set + PW_ * 4);
+  if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
+  if (myj >= 0) {
+#pragma unroll
+    for (int c = 0; c < PW_; ++c) oval[myj * PW_ + c] = (c < w) ? row[c] : cT{0, 0};
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_tslu_merge(int w, int nin, const char *wsin, char *wsout, long sW,
-                                                    long setb) {
+__global__ __launch_bounds__(TS_NT) void k_ts_merge(int c0, int w, int nin, const char *wsin, char *wsout, long sW,
+                                                    long setb, char *wsfin, int final_, int *ipiv, long sP, int *perm,
+                                                    int *info) {
   typedef typename C2<T>::t cT;
-  constexpr int PW = PanelW<T>::W;
-  extern __shared__ char smem_raw[];
-  cT *X = reinterpret_cast<cT *>(smem_raw);   // [2 PW][PW]
-  __shared__ int key[2 * PW], used[2 * PW], sel[PW], src[2 * PW], wr[2][4];
-  __shared__ T wv[2][4];
-  __shared__ int nrows_s;
+  __shared__ TsShared<T> sh;
   const int z = blockIdx.z, b = blockIdx.x, t = threadIdx.x;
-  const char *in = wsin + (long)z * sW;
+  const int si = b * TS_FAN + t / PW_, i = t % PW_;
+  const char *in = wsin + (long)z * sW + (long)si * setb;
+  int key = -1;
+  if (si < nin) key = reinterpret_cast<const int *>(in)[i];
+  const bool act = key >= 0;
+  const cT *row = reinterpret_cast<const cT *>(in + PW_ * 4) + (long)i * PW_;
+  cT x[PW_];
+#pragma unroll
+  for (int c = 0; c < PW_; ++c) x[c] = (act && c < w) ? row[c] : cT{0, 0};
+  int myj;
+  const int ns = ts_gepp<T>(x, act, key, w, sh, myj);
+  if (final_) {
+    ts_finish<T>(x, myj, ns, w, c0, sh, wsfin + (long)z * sW, ipiv + (long)z * sP,
+                 perm ? perm + (long)z * sP : nullptr, info + z);
+    return;
+  }
   char *out = wsout + (long)z * sW + (long)b * setb;
-  if (t == 0) {
-    int nr = 0;
-    for (int q = 0; q < 2; ++q) {
-      const int si = 2 * b + q;
-      if (si >= nin) break;
-      const int *idx = reinterpret_cast<const int *>(in + (long)si * setb);
-      for (int i = 0; i < PW; ++i)
-        if (idx[i] >= 0) { key[nr] = idx[i]; src[nr] = si * PW + i; ++nr; }
-    }
-    nrows_s = nr;
-  }
-  __syncthreads();
-  const int nr = nrows_s;
-  for (int e = t; e < nr * PW; e += 256) {
-    const int r = e / PW, c = e % PW;
-    const int si = src[r] / PW, i = src[r] % PW;
-    X[e] = reinterpret_cast<const cT *>(in + (long)si * setb + PW * 4)[i * PW + c];
-  }
-  __syncthreads();
-  const int ns = gepp_select<T>(X, nr, w, key, sel, used, wv, wr);
   int *oidx = reinterpret_cast<int *>(out);
-  cT *oval = reinterpret_cast<cT *>(out + PW * 4);
-  for (int i = t; i < PW; i += 256) oidx[i] = (i < ns) ? key[sel[i]] : -1;
-  for (int e = t; e < ns * PW; e += 256) {
-    const int i = e / PW, c = e % PW;
-    const int si = src[sel[i]] / PW, ii = src[sel[i]] % PW;
-    oval[e] = reinterpret_cast<const cT *>(in + (long)si * setb + PW * 4)[ii * PW + c];
+  cT *oval = reinterpret_cast<cT *>(out + PW_ * 4);
+  if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
+  if (myj >= 0) {
+#pragma unroll
+    for (int c = 0; c < PW_; ++c) oval[myj * PW_ + c] = row[c];
   }
 }
 
+// Row interchanges of one strip on every column of the matrix, as the
+// gather A[c0 + to] = A[c0 + from] over the moved rows (staged in LDS per
+// block of 64 columns), then L11\U11 into the top block.
+constexpr int AP_COLS = 32;
 template <class T>
-__global__ __launch_bounds__(256) void k_tslu_top(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
-                                                  const char *wsfin, long sW, int *ipiv, long sP, int *perm,
-                                                  int *info) {
+__global__ __launch_bounds__(256) void k_ts_apply(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
+                                                  const char *fin, long sW) {
   typedef typename C2<T>::t cT;
-  constexpr int PW = PanelW<T>::W;
-  __shared__ int tgt[PW], pos_of[2 * PW], row_of[2 * PW], ids[2 * PW], swp[PW];
-  __shared__ cT U[PW * PW];
-  __shared__ int nids;
-  const int z = blockIdx.z, t = threadIdx.x;
+  __shared__ cT buf[2 * PW_ * AP_COLS];
+  __shared__ int from[2 * PW_], to[2 * PW_];
+  const int z = blockIdx.z, t = threadIdx.x, j0 = blockIdx.x * AP_COLS;
   A += (long)z * sA;
-  ipiv += (long)z * sP;
-  if (perm) perm += (long)z * sP;
-  info += z;
-  const int *fidx = reinterpret_cast<const int *>(wsfin + (long)z * sW);
-  if (t < w) tgt[t] = fidx[t];
+  fin += (long)z * sW;
+  const cT *lu = reinterpret_cast<const cT *>(fin);
+  const int *lfrom = reinterpret_cast<const int *>(fin + PW_ * PW_ * sizeof(cT));
+  const int *lto = lfrom + 2 * PW_;
+  const int k = lto[2 * PW_];
+  if (t < k) { from[t] = lfrom[t]; to[t] = lto[t]; }
   __syncthreads();
-  for (int i = 0; i < w; ++i)
-    if (tgt[i] < 0 || tgt[i] >= n - c0) {   // cannot happen for n - c0 >= w; never index with it
-      if (t == 0) *info = -1;
-      return;
-    }
-  // interchange sequence: strip rows involved are 0..w-1 and the targets
-  if (t == 0) {
-    int k = 0;
-    for (int i = 0; i < w; ++i) { ids[k] = i; pos_of[k] = i; row_of[k] = i; ++k; }
-    for (int i = 0; i < w; ++i) {
-      int f = -1;
-      for (int q = 0; q < k; ++q) if (ids[q] == tgt[i]) { f = q; break; }
-      if (f < 0) { ids[k] = tgt[i]; pos_of[k] = tgt[i]; row_of[k] = tgt[i]; ++k; }
-    }
-    // ids[q]: original strip row; pos_of[q]: its current position
-    for (int i = 0; i < w; ++i) {
-      int qt = 0;
-      for (int q = 0; q < k; ++q) if (ids[q] == tgt[i]) { qt = q; break; }
-      const int p = pos_of[qt];
-      swp[i] = p;
-      if (p != i) {
-        int qi = 0;   // entry whose current position is i
-        for (int q = 0; q < k; ++q) if (pos_of[q] == i) { qi = q; break; }
-        pos_of[qi] = p;
-        pos_of[qt] = i;
-      }
-    }
-    nids = k;
-  }
+  const int c = t % AP_COLS, r4 = t / AP_COLS;
+  const bool colok = j0 + c < n;
+  for (int e = r4; e < k; e += 256 / AP_COLS)
+    if (colok) buf[e * AP_COLS + c] = A[(long)(c0 + from[e]) * lda + j0 + c];
   __syncthreads();
-  for (int i = 0; i < w; ++i) {
-    const int p = swp[i];
-    if (p != i) {
-      for (int c = t; c < w; c += 256) {
-        const cT a = A[(long)(c0 + i) * lda + c0 + c], b = A[(long)(c0 + p) * lda + c0 + c];
-        A[(long)(c0 + i) * lda + c0 + c] = b;
-        A[(long)(c0 + p) * lda + c0 + c] = a;
-      }
-    }
-    if (t == 0) {
-      ipiv[c0 + i] = c0 + p + 1;
-      if (perm && p != i) { const int a = perm[c0 + i]; perm[c0 + i] = perm[c0 + p]; perm[c0 + p] = a; }
-    }
-    __syncthreads();
-  }
-  // unpivoted LU of the top w x w block
-  for (int e = t; e < w * w; e += 256) U[(e / w) * PW + e % w] = A[(long)(c0 + e / w) * lda + c0 + e % w];
+  for (int e = r4; e < k; e += 256 / AP_COLS)
+    if (colok) A[(long)(c0 + to[e]) * lda + j0 + c] = buf[e * AP_COLS + c];
   __syncthreads();
-  for (int j = 0; j < w; ++j) {
-    const cT d = U[j * PW + j];
-    const bool zero = (d.x == 0 && d.y == 0);
-    if (t == 0 && zero && *info == 0) *info = c0 + j + 1;
-    if (!zero) {
-      for (int r = j + 1 + t; r < w; r += 256) U[r * PW + j] = cdiv<T>(U[r * PW + j], d);
-      __syncthreads();
-      const int nc = w - j - 1;
-      for (int e = t; e < nc * nc; e += 256) {
-        const int r = j + 1 + e / nc, c = j + 1 + e % nc;
-        const cT l = U[r * PW + j], u = U[j * PW + c];
-        cT &x = U[r * PW + c];
-        x = cT{x.x - (l.x * u.x - l.y * u.y), x.y - (l.x * u.y + l.y * u.x)};
-      }
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < w * w; e += 256) A[(long)(c0 + e / w) * lda + c0 + e % w] = U[(e / w) * PW + e % w];
-  (void)nids; (void)row_of;
+  if (colok && j0 + c >= c0 && j0 + c < c0 + w)
+    for (int i = r4; i < w; i += 256 / AP_COLS) A[(long)(c0 + i) * lda + j0 + c] = lu[i * PW_ + (j0 + c - c0)];
 }
 
 // L21 = A21 U11^{-1}: one thread per strip row below the top block.
@@ -669,14 +645,12 @@ constexpr int LU_NB = 256;   // outer block: K of the trailing GEMM
 // Workspace bytes for getrf_batched<T>(n, batch).
 template <class T>
 static size_t getrf_ws_bytes(int n, int batch) {
-  const int RPB = panel_rows_per_block<T>();
-  const int G = (n + RPB - 1) / RPB;
-  return (size_t)batch * ((PanelWS<T>::bytes(G) + 255) / 256 * 256);
+  return (size_t)batch * PanelWS<T>::bytes(n);
 }
 
 // Blocked right-looking LU with tournament-pivoted panels of `batch` n x n
 // matrices (stride sA), in place. Outer blocks of LU_NB columns; inside a
-// block the columns are factorised PW at a time (k_tslu_*), the block's
+// block the columns are factorised PW at a time (k_ts_*), the block's
 // remaining columns updated by TRSM + GEMM (K = PW); then U12 = L11^{-1} A12
 // blocked by PW and the trailing A22 -= L21 U12 (K = LU_NB).
 // ipiv (1-based, stride sP) lists the row interchanges as zgetrf's does;
@@ -686,14 +660,12 @@ static size_t getrf_ws_bytes(int n, int batch) {
 template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
                           int *perm, int *dinfo, hipStream_t s, char *ws = nullptr) {
-  typedef typename C2<T>::t cT;
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
   if (n <= 0 || batch <= 0) return;
-  const int RPB = panel_rows_per_block<T>();
-  const int Gmax = (n + RPB - 1) / RPB;
-  const long sW = (long)((PanelWS<T>::bytes(Gmax) + 255) / 256 * 256);
-  const long setb = (long)((PanelWS<T>::set_bytes() + 255) / 256 * 256);
+  const int Gmax = PanelWS<T>::sets(n);
+  const long sW = (long)PanelWS<T>::bytes(n);
+  const long setb = (long)PanelWS<T>::set_bytes();
   if (!ws) {
     static char *own = nullptr;
     static size_t own_bytes = 0;
@@ -706,28 +678,29 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     ws = own;
   }
   char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
+  char *fin = ws + 2 * (long)Gmax * setb;            // last level's output
+  const dim3 bts(TS_NT);
   if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
   for (int k0 = 0; k0 < n; k0 += LU_NB) {
     const int nb = min(LU_NB, n - k0);
     for (int j0 = 0; j0 < nb; j0 += PW) {
       const int w = min(PW, nb - j0), c0 = k0 + j0;
       // tournament-pivoted panel of the strip rows c0..n-1
-      const int G = (n - c0 + RPB - 1) / RPB;
-      hipLaunchKernelGGL(k_tslu_local<T>, dim3(G, 1, batch), b256, PANEL_LDS, s, n, c0, w, RPB, A, lda, sA, bufA, sW,
-                         setb);
+      const int G = (n - c0 + TS_NT - 1) / TS_NT;
+      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, 0, s, n, c0, w, A, lda, sA, ws, sW, setb,
+                         (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
       char *cur = bufA, *nxt = bufB;
-      for (int nsets = G; nsets > 1; nsets = (nsets + 1) / 2) {
-        hipLaunchKernelGGL(k_tslu_merge<T>, dim3((nsets + 1) / 2, 1, batch), b256,
-                           (size_t)2 * PW * PW * sizeof(cT), s, w, nsets, cur, nxt, sW, setb);
+      for (int nsets = G; nsets > 1; nsets = (nsets + TS_FAN - 1) / TS_FAN) {
+        const int nout = (nsets + TS_FAN - 1) / TS_FAN;
+        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, 0, s, c0, w, nsets, cur, nxt, sW, setb, fin,
+                           nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
         char *tmp = cur; cur = nxt; nxt = tmp;
       }
-      hipLaunchKernelGGL(k_tslu_top<T>, dim3(1, 1, batch), b256, 0, s, n, c0, w, A, lda, sA, cur, sW, ipiv, sP, perm,
-                         dinfo);
+      // the interchanges on every column, L11\U11 into place, then L21
+      hipLaunchKernelGGL(k_ts_apply<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, c0, w, A, lda, sA,
+                         fin, sW);
       if (n - c0 - w > 0)
         hipLaunchKernelGGL(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, s, n, c0, w, A, lda, sA);
-      // the strip's interchanges on every other column
-      if (n - w > 0)
-        hipLaunchKernelGGL(k_laswp<T>, dim3((n - w + 255) / 256, 1, batch), b256, 0, s, n, c0, w, A, lda, sA, ipiv, sP);
       // rest of this outer block: U part by TRSM, then the rank-w update
       const int rest = k0 + nb - (c0 + w);
       if (rest > 0) {

@@ -59,7 +59,8 @@ int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int *ipiv, int 
 int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
 		double *A0, double *A1, double *A0c, double *A1c, void *stream);
 int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
-		double rank_tol, size_t rank_sel_min, complex double *eigval, double *sigma, void *stream);
+		double rank_tol, size_t rank_sel_min, complex double *eigval, double *sigma, void *stream, void *ws);
+size_t qh_beyn_ws_bytes(size_t m, size_t l);
 int qh_e32_bq_mode(void);
 double l2d_unitcell_area(cart2_t b1, cart2_t b2);
 double *qh_collect(PGen *g, size_t *n, int dims);   /* ewald.c: PGen points as (x, y[, z]) */
@@ -170,7 +171,7 @@ typedef struct {
 	double *bV, *bM, *bX, *bR, *bW, *btd, *bpos;
 	float *bM32, *bR32, *bW32;
 	int *bipiv, *bperm, *binfo, *btt;
-	void *bws;
+	void *bws, *btail;
 	beyn_contour_t *bcont;
 	int active;
 } qh_unit;
@@ -236,9 +237,14 @@ static size_t beyn_node_bytes(const qpms_hip_server_t *s, size_t L, size_t l) {
 	return align256(L * L * 16) + (s->o.mixed ? align256(L * L * 8) : 0) + 4 * align256(L * l * 16)
 		+ 2 * align256(L * l * 8) + 2 * align256(L * 4) + 1024;
 }
-static size_t beyn_bytes(const qpms_hip_server_t *s, size_t L, size_t l, int chunk) {
+/* A contour of N nodes handled `chunk` nodes at a time: per-chunk matrices and
+ * workspaces, the probe and moment blocks, and per-node Mie parameters and
+ * T diagonals for all N nodes (lMax <= 4: nb <= 48). */
+static size_t beyn_bytes(const qpms_hip_server_t *s, size_t L, size_t l, int chunk, int N, int lMax) {
+	const size_t nb = 2 * (size_t)lMax * (lMax + 2);
 	return chunk * beyn_node_bytes(s, L, l) + align256(qh_getrf_ws_bytes(s->o.mixed ? 32 : 64, (int)L, chunk))
-		+ 5 * align256(L * l * 16) + align256(L * 24) + 65536 + chunk * 4096;
+		+ 5 * align256(L * l * 16) + align256(L * 24) + align256(qh_beyn_ws_bytes(L, l))
+		+ align256(N * 9 * 8) + align256(N * 2 * (lMax + 1) * 16) + align256(N * nb * 16) + 65536 + chunk * 4096;
 }
 
 static const qpms_trans_calculator *srv_calc(qpms_hip_server_t *s, int lMax) {
@@ -305,9 +311,10 @@ static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_c
 		case QPMS_HIP_REQ_BEYN: {
 			const size_t l = (size_t)j->req.beyn_l;
 			int c = j->req.beyn_nodes;
-			while (c > 1 && beyn_bytes(s, L, l, c) > s->arena.size / 2) --c;
+			const int N = j->req.beyn_nodes, lm = j->req.lMax;
+			while (c > 1 && beyn_bytes(s, L, l, c, N, lm) > s->arena.size / 2) --c;
 			if (beyn_chunk) *beyn_chunk = c;
-			return beyn_bytes(s, L, l, c);
+			return beyn_bytes(s, L, l, c, N, lm);
 		}
 	}
 	return 0;
@@ -581,7 +588,7 @@ static void launch_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int ch
 	const int lMax = r->lMax;
 	const size_t np = r->npart, L = jb->L, nb = jb->nb, l = (size_t)r->beyn_l;
 	const int N = r->beyn_nodes;
-	qh_bump bp = {s->arena.base + u->arena_off, beyn_bytes(s, L, l, chunk)};
+	qh_bump bp = {s->arena.base + u->arena_off, beyn_bytes(s, L, l, chunk, N, lMax)};
 	u->chunk = chunk;
 	u->next_node = 0;
 	u->bV = bump(&bp, L * l * 16);
@@ -597,18 +604,22 @@ static void launch_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int ch
 	u->btt = bump(&bp, np * 4);
 	double *dprm = bump(&bp, N * 9 * 8), *mie = bump(&bp, N * 2 * (lMax + 1) * 16);
 	u->btd = bump(&bp, N * nb * 16);
+	u->btail = bump(&bp, qh_beyn_ws_bytes(L, l));
 	/* staged: positions and the Mie parameters of every node */
 	const size_t in_bytes = align256(np * 24) + align256(N * 9 * 8);
 	u->pin_bytes = in_bytes + 64;
 	pin_take(s, u);
 	memcpy(u->pin, jb->pos, np * 24);
-	u->bcont = beyn_contour_ellipse(r->omega, r->beyn_rRe, r->beyn_rIm, (size_t)N);
+	u->bcont = beyn_contour_ellipse(CMPLX(r->omega, r->beyn_centre_im), r->beyn_rRe, r->beyn_rIm, (size_t)N);
 	double *hp = (double *)(u->pin + align256(np * 24));
 	for (int n = 0; n < N; ++n) mie_params(r, u->bcont->z_dz[n][0], hp + 9 * n);
 	SRV_HIPCHK(hipMemcpyAsync(u->bpos, u->pin, np * 24, hipMemcpyHostToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(dprm, hp, N * 9 * 8, hipMemcpyHostToDevice, st));
 	SRV_HIPCHK(hipMemsetAsync(u->btt, 0, np * 4, st));
-	SRV_HIPCHK(hipMemsetAsync(u->dA0, 0, 4 * L * l * 16, st));   /* A0, A1, A0c, A1c are contiguous */
+	SRV_HIPCHK(hipMemsetAsync(u->dA0, 0, L * l * 16, st));
+	SRV_HIPCHK(hipMemsetAsync(u->dA1, 0, L * l * 16, st));
+	SRV_HIPCHK(hipMemsetAsync(u->dA0c, 0, L * l * 16, st));
+	SRV_HIPCHK(hipMemsetAsync(u->dA1c, 0, L * l * 16, st));
 	qh_launch_probe((long)(L * l), r->beyn_seed, u->bV, st);
 	qh_launch_mie(lMax, N, dprm, mie, st);
 	qh_launch_expand_tdiag(lMax, N, mie, u->btd, st);
@@ -672,7 +683,9 @@ static void finish_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, qpms_h
 		snprintf(out->message, sizeof out->message, "cancelled after %d of %d nodes", u->next_node, r->beyn_nodes);
 		return;
 	}
-	const int K = qh_beyn_process_s(jb->L, l, u->dA0, u->dA1, r->omega, r->rank_tol, 1, out->eigval, out->sv, st);
+	const int K = qh_beyn_process_s(jb->L, l, u->dA0, u->dA1, CMPLX(r->omega, r->beyn_centre_im), r->rank_tol, 1,
+		out->eigval, out->sv, st,
+		u->btail);
 	out->neig = K;
 }
 

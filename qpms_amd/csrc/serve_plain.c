@@ -124,7 +124,8 @@ int qpms_hip_request_plain(const qpms_hip_request_t *r, qpms_hip_response_t *out
 		complex double *V;
 		QH_MALLOC(V, L * l * sizeof(complex double));
 		qh_probe_host((long)(L * l), r->beyn_seed, (double *)V);
-		beyn_contour_t *c = beyn_contour_ellipse(r->omega, r->beyn_rRe, r->beyn_rIm, (size_t)N);
+		beyn_contour_t *c = beyn_contour_ellipse(CMPLX(r->omega, r->beyn_centre_im), r->beyn_rRe, r->beyn_rIm,
+			(size_t)N);
 		struct qh_plain_beyn p = {ss};
 		beyn_result_t *br = beyn_solve_v(L, l, plain_beyn_M, NULL, &p, c, rank_tol, 1, 0.0, V);
 		out->neig = (int)br->neig;

@@ -10,7 +10,8 @@ background n = 1.52. 70 % plane-wave scattering solves (x-polarised, normal
 incidence), 20 % periodic mode-problem determinants (the cluster as the unit
 cell of a square lattice of its own extent, Bloch vector uniform in the first
 zone), 10 % Beyn contours (32 nodes, ellipse centred at omega with
-half-axes 0.05 eV and 0.005 eV, 32 probe columns).
+half-axes 0.05 eV and 0.005 eV, 32 probe columns, singular-value cut
+beyn_rank_tol).
 """
 import ctypes
 
@@ -29,7 +30,8 @@ class Request(ctypes.Structure):
                 ("eps_bg", ctypes.c_double), ("omega", ctypes.c_double), ("kdir", ctypes.c_double * 3),
                 ("E", c_cplx * 3), ("lattice", (ctypes.c_double * 2) * 2), ("beta", ctypes.c_double * 2),
                 ("beyn_rRe", ctypes.c_double), ("beyn_rIm", ctypes.c_double), ("beyn_nodes", ctypes.c_int),
-                ("beyn_l", ctypes.c_int), ("beyn_seed", ctypes.c_ulonglong), ("rank_tol", ctypes.c_double)]
+                ("beyn_l", ctypes.c_int), ("beyn_seed", ctypes.c_ulonglong), ("rank_tol", ctypes.c_double),
+                ("beyn_centre_im", ctypes.c_double)]
 
 
 class Response(ctypes.Structure):
@@ -127,7 +129,9 @@ class Req:
         c = Request()
         c.kind, c.lMax, c.npart = kind, lMax, len(self.pos)
         c.pos = self.pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        c.radius, c.eps_in, c.eps_bg, c.omega = radius, c_cplx.of(eps_in), eps_bg, omega
+        # omega may be complex for a Beyn request (the contour centre)
+        c.radius, c.eps_in, c.eps_bg = radius, c_cplx.of(eps_in), eps_bg
+        c.omega, c.beyn_centre_im = complex(omega).real, complex(omega).imag
         for i in range(3):
             c.kdir[i] = kdir[i]
             c.E[i] = c_cplx.of(E[i])
@@ -146,11 +150,36 @@ class Req:
 
     def clone(self, arrival=0.0):
         c = self.c
-        return Req(c.kind, self.pos, c.lMax, c.radius, complex(c.eps_in.re, c.eps_in.im), c.eps_bg, c.omega,
+        return Req(c.kind, self.pos, c.lMax, c.radius, complex(c.eps_in.re, c.eps_in.im), c.eps_bg,
+                   complex(c.omega, c.beyn_centre_im),
                    kdir=tuple(c.kdir), E=tuple(complex(e.re, e.im) for e in c.E),
                    lattice=((c.lattice[0][0], c.lattice[0][1]), (c.lattice[1][0], c.lattice[1][1])),
                    beta=(c.beta[0], c.beta[1]), beyn_rRe=c.beyn_rRe, beyn_rIm=c.beyn_rIm, beyn_nodes=c.beyn_nodes,
                    beyn_l=c.beyn_l, beyn_seed=c.beyn_seed, rank_tol=c.rank_tol, arrival=arrival)
+
+
+def beyn_rank_tol(dim, l, rRe, rel=1e-5):
+    """Absolute singular-value cut for a Beyn request, scaled to the size of
+    the contour integral: A0 = sum_n dz_n M(z_n)^-1 V has norm about
+    2 pi rRe sqrt(dim l) when an eigenvalue is enclosed, and rounding noise
+    about 1e-16 of that times cond(M). `rel` of the scale separates the two."""
+    return rel * 2 * np.pi * rRe * np.sqrt(dim * l)
+
+
+def beyn_rel_diff(a, b, rank_tol):
+    """Largest relative difference between two Beyn answers: the singular
+    values above the rank cut (relative to the largest) and the eigenvalues
+    (matched nearest-first, relative to |z|). inf if the counts differ."""
+    sa, sb = np.array(a.sv[:MAX_EIG]), np.array(b.sv[:MAX_EIG])
+    keep = sb > rank_tol
+    if keep.sum() != (sa > rank_tol).sum() or a.neig != b.neig:
+        return float("inf")
+    err = float(np.max(np.abs(sa[keep] - sb[keep])) / sb[0]) if keep.any() else 0.0
+    ea, eb = list(a.eigenvalues()), list(b.eigenvalues())
+    for z in eb:
+        j = int(np.argmin([abs(w - z) for w in ea]))
+        err = max(err, abs(ea.pop(j) - z) / abs(z))
+    return err
 
 
 def request_plain(req):
@@ -187,6 +216,7 @@ def serving_stream(n=10000, rate=20.0, seed=20261202, pitch=400e-9, n_bg=1.52):
             beta = rng.uniform(-np.pi / a, np.pi / a, size=2)
             out.append(Req(KIND_PERIODIC, lattice=((a, 0), (0, a)), beta=tuple(beta), **common))
         else:
+            dim = N * 2 * lMax * (lMax + 2)
             out.append(Req(KIND_BEYN, beyn_rRe=0.05 * EV, beyn_rIm=0.005 * EV, beyn_nodes=32, beyn_l=32,
-                           beyn_seed=1000 + i, **common))
+                           beyn_seed=1000 + i, rank_tol=beyn_rank_tol(dim, 32, 0.05 * EV), **common))
     return out

@@ -1,6 +1,8 @@
-"""GPU: the blocked LU (cooperative tall-strip panels, rank-256 trailing
-updates) in FP64 and FP32, single and batched, across the outer-block and
-strip boundaries, against P A = L U, numpy solves and zgetrf's info."""
+"""GPU: the blocked LU (tournament-pivoted tall-strip panels, rank-256
+trailing updates) in FP64 and FP32, single and batched, across the
+outer-block and strip boundaries and the tournament's one- and two-level
+merges (strips of more than 16 x 512 rows), against P A = L U, numpy solves
+and zgetrf's info."""
 import ctypes
 
 import numpy as np
@@ -32,7 +34,7 @@ def check_plu(A, LU, piv, tol, nprobe=4):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("n", [63, 64, 65, 255, 256, 257, 513, 1500, 3000])
+@pytest.mark.parametrize("n", [63, 64, 65, 255, 256, 257, 511, 512, 513, 1500, 3000, 8700])
 def test_zgetrf_blocks(gpu, n):
     import qpms_amd
     A = rand(n, n)

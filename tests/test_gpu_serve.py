@@ -1,10 +1,11 @@
 """GPU: the serving loop (include/qpms_hip_serve.h) against the plain FP64
 drop-in path (the public scattering-system API, one request at a time) on a
 small mixed stream: plane-wave solves of several sizes (batched and
-unbatched), 2D-periodic determinants and Beyn contours. Each result must
-agree with the plain path (extinction and log-determinant to 1e-9 relative,
-Beyn singular values to 1e-9 of the largest) and be the same when the request
-runs alone as when it runs in the mix (1e-12)."""
+unbatched), 2D-periodic determinants and Beyn contours around resonances
+of the clusters. Each result must agree with the plain path (extinction and
+log-determinant to 1e-9 relative; Beyn eigenfrequencies to 1e-9 relative
+and the singular values above the rank cut to 1e-9 of the largest) and be
+the same when the request runs alone as when it runs in the mix (1e-12)."""
 import numpy as np
 import pytest
 
@@ -34,10 +35,13 @@ def requests():
         pos, a = grid(N, 400e-9, rng)
         out.append(sv.Req(sv.KIND_PERIODIC, pos, lMax, 50e-9, -12 + 1j, 1.52 ** 2, 2.0 * EV,
                           lattice=((a, 0), (0, a)), beta=tuple(rng.uniform(-np.pi / a, np.pi / a, 2))))
-    for N, lMax in [(4, 2), (9, 2)]:
+    # contours around resonances of these clusters (found with the oracle:
+    # 3 and 2 eigenfrequencies inside, the rest at least 0.04 eV outside)
+    for (N, lMax), (c, rre, rim) in zip([(4, 2), (9, 2)], [(1.12 - 0.25j, 0.1, 0.05), (1.204 - 0.143j, 0.05, 0.04)]):
         pos, _ = grid(N, 400e-9, rng)
-        out.append(sv.Req(sv.KIND_BEYN, pos, lMax, 70e-9, -12 + 1j, 1.52 ** 2, 2.05 * EV, beyn_rRe=0.05 * EV,
-                          beyn_rIm=0.005 * EV, beyn_nodes=16, beyn_l=16, beyn_seed=7 + N))
+        out.append(sv.Req(sv.KIND_BEYN, pos, lMax, 70e-9, -12 + 1j, 1.52 ** 2, c * EV, beyn_rRe=rre * EV,
+                          beyn_rIm=rim * EV, beyn_nodes=16, beyn_l=16, beyn_seed=7 + N,
+                          rank_tol=sv.beyn_rank_tol(N * 2 * lMax * (lMax + 2), 16, rre * EV)))
     return out
 
 
@@ -50,7 +54,7 @@ def drain(srv):
     return got
 
 
-def close(a, b, kind, tol):
+def close(a, b, kind, tol, req=None):
     from qpms_amd import serve as sv
     assert a.status == 0 and b.status == 0, (a.message, b.message)
     assert a.dim == b.dim
@@ -61,8 +65,7 @@ def close(a, b, kind, tol):
         d = (a.argdet - b.argdet + np.pi) % (2 * np.pi) - np.pi
         assert abs(d) <= tol * max(1.0, abs(b.logabsdet)) + 1e-12
     else:
-        sa, sb = np.array(a.sv[:16]), np.array(b.sv[:16])
-        assert np.max(np.abs(sa - sb)) <= tol * sb[0], (sa, sb)
+        assert sv.beyn_rel_diff(a, b, req.c.rank_tol) <= tol, (a.sv[:4], b.sv[:4], a.eigenvalues(), b.eigenvalues())
 
 
 def test_serving_matches_plain_alone_and_mix(gpu):
@@ -79,10 +82,45 @@ def test_serving_matches_plain_alone_and_mix(gpu):
         alone.append(drain(srv)[rid])
     srv.close()
     for r, p, i, a in zip(reqs, plain, ids, alone):
-        close(mix[i], p, r.kind, 1e-9)
-        close(mix[i], a, r.kind, 1e-12)
+        close(mix[i], p, r.kind, 1e-9, r)
+        close(mix[i], a, r.kind, 1e-12, r)
         if r.kind == sv.KIND_SCATTER:
             assert mix[i].rel_residual < 1e-14
+
+
+def test_served_beyn_matches_oracle(gpu):
+    """The served contours against the oracle's Beyn (numpy restatement of
+    beyn.c with the oracle's M = I - T W at every node), same probe."""
+    import ctypes
+    import oracle_beyn as ob
+    import oracle_ffi as of
+    from qpms_amd import serve as sv
+    reqs = [r for r in requests() if r.kind == sv.KIND_BEYN]
+    srv = sv.Server(arena_gib=4, nstreams=2, mixed=True)
+    ids = [srv.submit(r) for r in reqs]
+    got = drain(srv)
+    srv.close()
+    lib = sv.lib()
+    lib.qh_probe_host.argtypes = [ctypes.c_long, ctypes.c_ulonglong, ctypes.c_void_p]
+    for r, i in zip(reqs, ids):
+        c = r.c
+        N, lMax, L, l = c.npart, c.lMax, r.dim, c.beyn_l
+        oc = of.OracleCalc(lMax)
+        eps_in = complex(c.eps_in.re, c.eps_in.im)
+
+        def Mf(z):
+            td, k = of.mie_tdiag(lMax, c.radius, eps_in, c.eps_bg, z)
+            return oc.build_M(r.pos, np.zeros(N, np.int32), td, k)
+        V = np.empty((L, l), np.complex128)
+        lib.qh_probe_host(L * l, c.beyn_seed, V.ctypes.data)
+        centre = complex(c.omega, c.beyn_centre_im)
+        z, dz = ob.contour_ellipse(centre, c.beyn_rRe, c.beyn_rIm, c.beyn_nodes)
+        ev, _, _, S, _ = ob.beyn_solve(Mf, L, V, z, dz, centre, rank_tol=c.rank_tol, rank_sel_min=1)
+        g = got[i]
+        assert g.status == 0 and g.neig == len(ev) and g.neig >= 2
+        keep = S > c.rank_tol
+        assert np.max(np.abs(np.array(g.sv[:l])[keep] - S[keep])) <= 1e-9 * S[0]
+        assert ob.match_sets(g.eigenvalues(), ev) <= 1e-9
 
 
 def test_serving_fp64_mode_matches_plain(gpu):
