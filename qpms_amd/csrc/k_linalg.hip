@@ -8,8 +8,9 @@
 //
 // LU structure (getrf_batched): outer blocks of LU_NB = 256 columns.
 //   k_ts_*        factor a tall strip of PW = 32 columns with tournament
-//                 pivoting: candidate rows chosen per 512-row chunk (one row
-//                 per thread, in registers), merged 16 sets at a time, the
+//                 pivoting: candidate rows chosen per chunk of 512 (FP32) or
+//                 256 (FP64) rows staged in LDS, merged a chunk's worth of
+//                 candidate sets at a time, the
 //                 winners moved to the top (recorded as zgetrf-style
 //                 interchanges) in every column by one gather, then L21 by a
 //                 triangular solve; no workgroup waits for another;
@@ -294,12 +295,12 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 // ---------------------------------------------------------------------------
 // Tall-strip panel: LU with tournament pivoting (CALU / TSLU, Grigori,
 // Demmel & Xiang 2011). The strip (rows c0..n-1, columns c0..c0+w-1,
-// w <= PW) is split into chunks of TS_NT rows, one row per thread held in
-// registers:
+// w <= PW) is split into chunks of R rows (512 in FP32, 256 in FP64) staged
+// in LDS, one row per thread:
 //   k_ts_local   each workgroup runs partial-pivoting elimination on its
 //                chunk only to choose w candidate rows, and writes their
 //                indices and original values;
-//   k_ts_merge   up to TS_FAN candidate sets per workgroup are merged the
+//   k_ts_merge   up to R / PW candidate sets per workgroup are merged the
 //                same way, level by level, until one set remains;
 //   the last level (local when one chunk covers the strip, else the last
 //                merge) also writes the w x w block L11\U11 as its
@@ -312,11 +313,21 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 // No workgroup ever waits for another one; each step is its own launch.
 // ---------------------------------------------------------------------------
 constexpr int PW_ = 32;           // strip width (FP32 and FP64)
-constexpr int TS_NT = 512;        // rows per tournament workgroup
-constexpr int TS_WAVES = TS_NT / 64;
-constexpr int TS_FAN = TS_NT / PW_;   // candidate sets merged per workgroup
+constexpr int TS_THREADS = 512;   // threads per tournament workgroup
+constexpr int TS_WAVES = TS_THREADS / 64;
+// LDS row stride (complex): conflict-free row-per-lane access with 16-byte
+// lane accesses (two FP32 complex values, or one FP64)
+template <class T> struct TsStride;
+template <> struct TsStride<float> { static constexpr int XS = PW_ + 2; };
+template <> struct TsStride<double> { static constexpr int XS = PW_ + 1; };
 
 template <class T> struct PanelW { static constexpr int W = PW_; };
+// rows per tournament workgroup: the chunk is staged in LDS (135 KB)
+template <class T> struct TsRows;
+template <> struct TsRows<float> { static constexpr int R = 512; };
+template <> struct TsRows<double> { static constexpr int R = 256; };
+template <class T> constexpr int ts_fan() { return TsRows<T>::R / PW_; }   // candidate sets merged per workgroup
+template <class T> constexpr size_t ts_lds() { return (size_t)TsRows<T>::R * TsStride<T>::XS * sizeof(typename C2<T>::t); }
 
 // Per matrix: two candidate buffers of Gmax sets (index + PW values each),
 // then the last level's output: L11\U11, the moved-row list and its length.
@@ -324,79 +335,160 @@ template <class T>
 struct PanelWS {
   static size_t set_bytes() { return ((size_t)PW_ * 4 + (size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 255) / 256 * 256; }
   static size_t fin_bytes() { return ((size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 4 * PW_ * 4 + 16 + 255) / 256 * 256; }
-  static int sets(int n) { return (n + TS_NT - 1) / TS_NT; }
+  static int sets(int n) { return (n + TsRows<T>::R - 1) / TsRows<T>::R; }
   static size_t bytes(int n) { return (2 * (size_t)sets(n) * set_bytes() + fin_bytes() + 255) / 256 * 256; }
 };
 
 template <class T>
 struct TsShared {
   T wv[2][TS_WAVES];
-  int wk[2][TS_WAVES], wt[2][TS_WAVES];
-  typename C2<T>::t prow[2][PW_];
+  int wk[2][TS_WAVES], wr[2][TS_WAVES];
+  int sel[PW_];    // block row of pivot j
   int skey[PW_];   // key (strip row) of pivot j
   T sval[PW_];     // |U_jj|_1 at selection
 };
 
-// Partial-pivoting elimination over the rows of a workgroup, row t in the
-// registers x[] of thread t. Per column: a wave-shuffle arg-max of |x_j|_1
-// (ties: smaller key, as izamax's first maximum), one LDS combine of the
-// waves, the pivot row broadcast through LDS, and the rank-1 update of the
-// thread's own row (two barriers). Returns the number of pivots chosen; on
-// return myj is the pivot order of this thread's row (-1 if not chosen) and
-// its x[] holds that row of L\U.
+// Arg-max over a wave of (a, key, row) with ties to the smaller key, by DPP
+// moves only: xor 1, xor 2, rotate 4, rotate 8 inside each 16-lane row, then
+// the row broadcasts; every lane returns lane 63's result.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v) { return __int_as_float(dpp_i<CTRL>(__float_as_int(v))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float rdlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <class T>
-__device__ __forceinline__ int ts_gepp(typename C2<T>::t (&x)[PW_], bool act, int key, int w, TsShared<T> &sh, int &myj) {
+__device__ __forceinline__ void amax_pick(T &a, int &k, int &r, T a2, int k2, int r2) {
+  const bool b = (a2 > a) | ((a2 == a) & (k2 < k));
+  a = b ? a2 : a;
+  k = b ? k2 : k;
+  r = b ? r2 : r;
+}
+template <class T, int CTRL>
+__device__ __forceinline__ void amax_dpp(T &a, int &k, int &r) {
+  amax_pick<T>(a, k, r, dpp_t<CTRL>(a), dpp_i<CTRL>(k), dpp_i<CTRL>(r));
+}
+// Row broadcasts (lane 15 of each row into the next row, lane 31 into the
+// upper half) for the rows the mask enables; disabled lanes keep `old`.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dppm_i(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xF, false); }
+template <int CTRL, int RM>
+__device__ __forceinline__ float dppm_t(float old, float v) {
+  return __int_as_float(dppm_i<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dppm_t(double old, double v) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = dppm_i<CTRL, RM>((int)o, (int)b), hi = dppm_i<CTRL, RM>((int)(o >> 32), (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T, int CTRL, int RM>
+__device__ __forceinline__ void amax_bcast(T &a, int &k, int &r) {
+  amax_pick<T>(a, k, r, dppm_t<CTRL, RM>(T(-2), a), dppm_i<CTRL, RM>(0x7fffffff, k), dppm_i<CTRL, RM>(0, r));
+}
+template <class T>
+__device__ __forceinline__ void wave_amax(T &a, int &k, int &r) {
+  amax_dpp<T, 0xB1>(a, k, r);         // quad_perm [1,0,3,2]
+  amax_dpp<T, 0x4E>(a, k, r);         // quad_perm [2,3,0,1]
+  amax_dpp<T, 0x124>(a, k, r);        // row_ror 4
+  amax_dpp<T, 0x128>(a, k, r);        // row_ror 8: every lane of a row holds the row's winner
+  amax_bcast<T, 0x142, 0xA>(a, k, r); // row_bcast 15: rows 1, 3 take rows 0, 2
+  amax_bcast<T, 0x143, 0xC>(a, k, r); // row_bcast 31: rows 2, 3 take row 1; lane 63 holds the wave's
+  a = rdlane(a, 63);
+  k = __builtin_amdgcn_readlane(k, 63);
+  r = __builtin_amdgcn_readlane(r, 63);
+}
+
+// Partial-pivoting elimination over the R rows of a workgroup staged in LDS
+// (X[row][XS]), used only to choose pivot rows. Row i belongs to thread i
+// (keys distinct, < 0 for an empty row). Per column: the wave arg-max of
+// |x_j|_1 (ties: smaller key, as izamax's first maximum), one LDS combine of
+// the waves (one barrier; the combine slots alternate between columns),
+// then every live row is reduced against the pivot row with the pivot's
+// reciprocal. A row's owner is the only writer of it, and the pivot row is
+// not written once chosen, so the next column's barrier orders the updates.
+// The loops stay rolled: the workgroup runs them once, in lock step. On
+// return sh.sel/skey/sval hold the pivots (rows of L\U in X for the chosen
+// rows); returns their number.
+// Y[c] -= l P[c] for c in [c1, w): one complex value per step, or (FP32)
+// pairs of them as 16-byte LDS accesses.
+__device__ __forceinline__ void ts_axpy(v2f *Y, const v2f *P, v2f l, int c1, int w) {
+  int c = c1;
+  if ((c & 1) && c < w) {
+    const v2f u = P[c], y = Y[c];
+    Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
+    ++c;
+  }
+#pragma unroll 2
+  for (; c + 1 < w; c += 2) {
+    const v4f u = *reinterpret_cast<const v4f *>(P + c);
+    v4f y = *reinterpret_cast<v4f *>(Y + c);
+    y.x -= l.x * u.x - l.y * u.y;
+    y.y -= l.x * u.y + l.y * u.x;
+    y.z -= l.x * u.z - l.y * u.w;
+    y.w -= l.x * u.w + l.y * u.z;
+    *reinterpret_cast<v4f *>(Y + c) = y;
+  }
+  if (c < w) {
+    const v2f u = P[c], y = Y[c];
+    Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
+  }
+}
+__device__ __forceinline__ void ts_axpy(v2d *Y, const v2d *P, v2d l, int c1, int w) {
+#pragma unroll 4
+  for (int c = c1; c < w; ++c) {
+    const v2d u = P[c], y = Y[c];
+    Y[c] = v2d{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
+  }
+}
+
+template <class T>
+__device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsShared<T> &sh) {
   typedef typename C2<T>::t cT;
+  constexpr int R = TsRows<T>::R, XS = TsStride<T>::XS;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  bool live = act, done = false;
-  myj = -1;
+  bool live = (t < R) && key >= 0;
   int ns = 0;
-#pragma clang loop unroll(full)
-  for (int j = 0; j < PW_; ++j) {
-    if (done || j >= w) continue;   // uniform
+  for (int j = 0; j < w; ++j) {
     const int par = j & 1;
+    cT xj = cT{0, 0};
     T a = -1;
     if (live) {
-      a = fabs(x[j].x) + fabs(x[j].y);
+      xj = X[t * XS + j];
+      a = fabs(xj.x) + fabs(xj.y);
       if (!(a == a)) a = 0;   // NaN: chosen only when nothing else is left
     }
-    int k = live ? key : 0x7fffffff, tid = t;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const T a2 = __shfl_xor(a, off);
-      const int k2 = __shfl_xor(k, off), t2 = __shfl_xor(tid, off);
-      if (a2 > a || (a2 == a && k2 < k)) { a = a2; k = k2; tid = t2; }
-    }
-    if (lane == 0) { sh.wv[par][wave] = a; sh.wk[par][wave] = k; sh.wt[par][wave] = tid; }
+    int k = live ? key : 0x7fffffff, r = t;
+    wave_amax<T>(a, k, r);
+    if (lane == 0) { sh.wv[par][wave] = a; sh.wk[par][wave] = k; sh.wr[par][wave] = r; }
     __syncthreads();
-    T pa = -1;
-    int pk = 0x7fffffff, pt = -1;
+    T pa = sh.wv[par][0];
+    int pk = sh.wk[par][0], pr = sh.wr[par][0];
 #pragma unroll
-    for (int q = 0; q < TS_WAVES; ++q) {
-      const T v = sh.wv[par][q];
-      const int kq = sh.wk[par][q];
-      if (v > pa || (v == pa && kq < pk)) { pa = v; pk = kq; pt = sh.wt[par][q]; }
-    }
-    if (pa < 0) { done = true; continue; }   // no row left (uniform)
-    if (t == pt) {
-#pragma unroll
-      for (int c = 0; c < PW_; ++c) sh.prow[par][c] = x[c];
-      live = false;
-      myj = j;
-    }
-    if (t == 0) { sh.skey[j] = pk; sh.sval[j] = pa; }
-    __syncthreads();
+    for (int q = 1; q < TS_WAVES; ++q) amax_pick<T>(pa, pk, pr, sh.wv[par][q], sh.wk[par][q], sh.wr[par][q]);
+    if (pa < 0) break;   // no row left (uniform)
+    if (t == pr) live = false;
+    if (t == 0) { sh.sel[j] = pr; sh.skey[j] = pk; sh.sval[j] = pa; }
     ns = j + 1;
     if (live && pa > 0) {
-      const cT l = cdiv<T>(x[j], sh.prow[par][j]);
-      x[j] = l;
-#pragma unroll
-      for (int c = j + 1; c < PW_; ++c) {
-        const cT u = sh.prow[par][c];
-        x[c] = cT{x[c].x - (l.x * u.x - l.y * u.y), x[c].y - (l.x * u.y + l.y * u.x)};
-      }
+      const cT *P = X + pr * XS;
+      cT *Y = X + t * XS;
+      const cT l = cmul<T>(xj, cdiv<T>(cT{1, 0}, P[j]));
+      Y[j] = l;
+      ts_axpy(Y, P, l, j + 1, w);
     }
   }
+  __syncthreads();
   return ns;
 }
 
@@ -406,17 +498,18 @@ __device__ __forceinline__ int ts_gepp(typename C2<T>::t (&x)[PW_], bool act, in
 // i <-> p is applied to it with wave ballots (a row not yet listed sits at
 // its own index).
 template <class T>
-__device__ __forceinline__ void ts_finish(const typename C2<T>::t (&x)[PW_], int myj, int ns, int w, int c0, TsShared<T> &sh,
-                          char *fin, int *ipiv, int *perm, int *info) {
+__device__ __forceinline__ void ts_finish(const typename C2<T>::t *X, int ns, int w, int c0, TsShared<T> &sh,
+                                          char *fin, int *ipiv, int *perm, int *info) {
   typedef typename C2<T>::t cT;
+  constexpr int XS = TsStride<T>::XS;
   const int t = threadIdx.x;
   cT *lu = reinterpret_cast<cT *>(fin);
   int *lfrom = reinterpret_cast<int *>(fin + PW_ * PW_ * sizeof(cT));
   int *lto = lfrom + 2 * PW_;
   int *lcount = lto + 2 * PW_;
-  if (myj >= 0) {
-#pragma unroll
-    for (int c = 0; c < PW_; ++c) lu[myj * PW_ + c] = (c < w) ? x[c] : cT{0, 0};
+  for (int e = t; e < PW_ * PW_; e += TS_THREADS) {
+    const int i = e / PW_, c = e % PW_;
+    lu[e] = (i < ns && c < w) ? X[sh.sel[i] * XS + c] : cT{0, 0};
   }
   if (t < 64) {
     int id = (t < w) ? t : -1, pos = id, nlist = w;
@@ -436,10 +529,13 @@ __device__ __forceinline__ void ts_finish(const typename C2<T>::t (&x)[PW_], int
           ++nlist;
         }
       }
-      if (t == 0) {
-        ipiv[c0 + i] = c0 + p + 1;
-        if (sh.sval[i] == 0 && *info == 0) *info = c0 + i + 1;
-      }
+      if (t == 0) ipiv[c0 + i] = c0 + p + 1;
+    }
+    if (t == 0) {
+      int z0 = 0;
+      for (int i = ns - 1; i >= 0; --i)
+        if (sh.sval[i] == 0) z0 = c0 + i + 1;
+      if (z0 && *info == 0) *info = z0;
     }
     const bool mv = (t < nlist) && id != pos;
     const unsigned long long mm = __ballot(mv);
@@ -454,67 +550,71 @@ __device__ __forceinline__ void ts_finish(const typename C2<T>::t (&x)[PW_], int
 }
 
 template <class T>
-__global__ __launch_bounds__(TS_NT) void k_ts_local(int n, int c0, int w, const typename C2<T>::t *A, long lda,
-                                                    long sA, char *ws, long sW, long setb, long finoff, int final_,
-                                                    int *ipiv, long sP, int *perm, int *info) {
+__global__ __launch_bounds__(TS_THREADS) void k_ts_local(int n, int c0, int w, const typename C2<T>::t *A, long lda,
+                                                         long sA, char *ws, long sW, long setb, long finoff,
+                                                         int final_, int *ipiv, long sP, int *perm, int *info) {
   typedef typename C2<T>::t cT;
+  constexpr int R = TsRows<T>::R, XS = TsStride<T>::XS;
+  extern __shared__ __align__(16) char smem_raw[];
+  cT *X = reinterpret_cast<cT *>(smem_raw);
   __shared__ TsShared<T> sh;
   const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x;
   A += (long)z * sA;
   ws += (long)z * sW;
-  const int m = n - c0, r = g * TS_NT + t;
-  const bool act = r < m;
-  cT x[PW_];
-  const cT *row = A + (long)(c0 + (act ? r : 0)) * lda + c0;
-#pragma unroll
-  for (int c = 0; c < PW_; ++c) x[c] = (act && c < w) ? row[c] : cT{0, 0};
-  int myj;
-  const int ns = ts_gepp<T>(x, act, r, w, sh, myj);
+  const int m = n - c0, r0 = g * R, nr = min(R, m - r0);
+  for (int e = t; e < R * PW_; e += TS_THREADS) {   // coalesced along each row
+    const int r = e / PW_, c = e % PW_;
+    X[r * XS + c] = (r < nr && c < w) ? A[(long)(c0 + r0 + r) * lda + c0 + c] : cT{0, 0};
+  }
+  __syncthreads();
+  const int ns = ts_gepp<T>(X, t < nr ? r0 + t : -1, w, sh);
   if (final_) {
-    ts_finish<T>(x, myj, ns, w, c0, sh, ws + finoff, ipiv + (long)z * sP, perm ? perm + (long)z * sP : nullptr,
-                 info + z);
+    ts_finish<T>(X, ns, w, c0, sh, ws + finoff, ipiv + (long)z * sP, perm ? perm + (long)z * sP : nullptr, info + z);
     return;
   }
   char *set = ws + (long)g * setb;
   int *oidx = reinterpret_cast<int *>(set);
   cT *oval = reinterpret_cast<cT *>(set + PW_ * 4);
   if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
-  if (myj >= 0) {
-#pragma unroll
-    for (int c = 0; c < PW_; ++c) oval[myj * PW_ + c] = (c < w) ? row[c] : cT{0, 0};
+  for (int e = t; e < ns * PW_; e += TS_THREADS) {   // the chosen rows' original values
+    const int i = e / PW_, c = e % PW_;
+    oval[e] = (c < w) ? A[(long)(c0 + sh.skey[i]) * lda + c0 + c] : cT{0, 0};
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(TS_NT) void k_ts_merge(int c0, int w, int nin, const char *wsin, char *wsout, long sW,
-                                                    long setb, char *wsfin, int final_, int *ipiv, long sP, int *perm,
-                                                    int *info) {
+__global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin, const char *wsin, char *wsout,
+                                                         long sW, long setb, char *wsfin, int final_, int *ipiv,
+                                                         long sP, int *perm, int *info) {
   typedef typename C2<T>::t cT;
+  constexpr int R = TsRows<T>::R, F = ts_fan<T>(), XS = TsStride<T>::XS;
+  extern __shared__ __align__(16) char smem_raw[];
+  cT *X = reinterpret_cast<cT *>(smem_raw);
   __shared__ TsShared<T> sh;
   const int z = blockIdx.z, b = blockIdx.x, t = threadIdx.x;
-  const int si = b * TS_FAN + t / PW_, i = t % PW_;
-  const char *in = wsin + (long)z * sW + (long)si * setb;
+  const char *in = wsin + (long)z * sW;
+  const int s0 = b * F, nset = min(F, nin - s0);
+  for (int e = t; e < R * PW_; e += TS_THREADS) {
+    const int r = e / PW_, c = e % PW_, si = r / PW_, i = r % PW_;
+    X[r * XS + c] = (si < nset) ? reinterpret_cast<const cT *>(in + (long)(s0 + si) * setb + PW_ * 4)[i * PW_ + c]
+                                : cT{0, 0};
+  }
   int key = -1;
-  if (si < nin) key = reinterpret_cast<const int *>(in)[i];
-  const bool act = key >= 0;
-  const cT *row = reinterpret_cast<const cT *>(in + PW_ * 4) + (long)i * PW_;
-  cT x[PW_];
-#pragma unroll
-  for (int c = 0; c < PW_; ++c) x[c] = (act && c < w) ? row[c] : cT{0, 0};
-  int myj;
-  const int ns = ts_gepp<T>(x, act, key, w, sh, myj);
+  if (t < R && t / PW_ < nset) key = reinterpret_cast<const int *>(in + (long)(s0 + t / PW_) * setb)[t % PW_];
+  __syncthreads();
+  const int ns = ts_gepp<T>(X, key, w, sh);
   if (final_) {
-    ts_finish<T>(x, myj, ns, w, c0, sh, wsfin + (long)z * sW, ipiv + (long)z * sP,
-                 perm ? perm + (long)z * sP : nullptr, info + z);
+    ts_finish<T>(X, ns, w, c0, sh, wsfin + (long)z * sW, ipiv + (long)z * sP, perm ? perm + (long)z * sP : nullptr,
+                 info + z);
     return;
   }
   char *out = wsout + (long)z * sW + (long)b * setb;
   int *oidx = reinterpret_cast<int *>(out);
   cT *oval = reinterpret_cast<cT *>(out + PW_ * 4);
   if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
-  if (myj >= 0) {
-#pragma unroll
-    for (int c = 0; c < PW_; ++c) oval[myj * PW_ + c] = row[c];
+  for (int e = t; e < ns * PW_; e += TS_THREADS) {   // the chosen rows' original values, from their input set
+    const int i = e / PW_, c = e % PW_, r = sh.sel[i];
+    oval[e] = reinterpret_cast<const cT *>(in + (long)(s0 + r / PW_) * setb + PW_ * 4)[(r % PW_) * PW_ + c];
   }
 }
 
@@ -679,21 +779,28 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   }
   char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
   char *fin = ws + 2 * (long)Gmax * setb;            // last level's output
-  const dim3 bts(TS_NT);
+  const dim3 bts(TS_THREADS);
+  static bool lds_set = false;   // the tournament kernels stage a chunk of more than 64 KB
+  if (!lds_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_ts_local<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
+    HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
+    lds_set = true;
+  }
   if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
   for (int k0 = 0; k0 < n; k0 += LU_NB) {
     const int nb = min(LU_NB, n - k0);
     for (int j0 = 0; j0 < nb; j0 += PW) {
       const int w = min(PW, nb - j0), c0 = k0 + j0;
       // tournament-pivoted panel of the strip rows c0..n-1
-      const int G = (n - c0 + TS_NT - 1) / TS_NT;
-      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, 0, s, n, c0, w, A, lda, sA, ws, sW, setb,
+      constexpr int F = ts_fan<T>();
+      const int G = (n - c0 + TsRows<T>::R - 1) / TsRows<T>::R;
+      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), s, n, c0, w, A, lda, sA, ws, sW, setb,
                          (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
       char *cur = bufA, *nxt = bufB;
-      for (int nsets = G; nsets > 1; nsets = (nsets + TS_FAN - 1) / TS_FAN) {
-        const int nout = (nsets + TS_FAN - 1) / TS_FAN;
-        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, 0, s, c0, w, nsets, cur, nxt, sW, setb, fin,
-                           nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
+      for (int nsets = G; nsets > 1; nsets = (nsets + F - 1) / F) {
+        const int nout = (nsets + F - 1) / F;
+        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), s, c0, w, nsets, cur, nxt, sW, setb,
+                           fin, nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
         char *tmp = cur; cur = nxt; nxt = tmp;
       }
       // the interchanges on every column, L11\U11 into place, then L21
