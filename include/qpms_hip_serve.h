@@ -58,7 +58,7 @@ typedef struct qpms_hip_request_t {
 	int beyn_nodes;
 	int beyn_l;
 	unsigned long long beyn_seed;
-	double rank_tol;            /* BEYN: absolute singular-value cut (beyn.c:382) */
+	double rank_tol;            /* BEYN: absolute singular-value cut (beyn.c:382); no forced minimum rank */
 	double beyn_centre_im;      /* BEYN: imaginary part of the contour centre (rad/s) */
 } qpms_hip_request_t;
 
@@ -72,7 +72,7 @@ typedef struct qpms_hip_response_t {
 	int refine_steps;
 	double logabsdet;           /* PERIODIC_DET: log |det M| */
 	double argdet;              /* PERIODIC_DET: arg det M in (-pi, pi] */
-	int neig;                   /* BEYN */
+	int neig;                   /* BEYN: eigenvalues whose singular value exceeds rank_tol */
 	_Complex double eigval[QPMS_HIP_SERVE_MAX_EIG];
 	double sv[QPMS_HIP_SERVE_MAX_EIG];  /* BEYN: singular values of A0 (beyn_l of them) */
 	double t_submit, t_start, t_done;   /* seconds on the server's clock */

@@ -110,6 +110,25 @@ int qpms_hip_ewald3_sigma_batch_dev(const qpms_ewald3_constants_t *c, double eta
 	return 0;
 }
 
+/* Long-range sums of ntrip triples sharing (beta, k): per-point terms once,
+ * then one phase sum per shift. F: qh_sigma_long_shifts_scratch(2 lMax + 1,
+ * nK) bytes of device scratch. */
+size_t qh_sigma_long_shifts_scratch(int LE, int npts);
+int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev, double eta,
+		double area, const double *leg0_dev, const double *s1_dev, const int *s1off_dev, double *F_dev,
+		double *out_dev, void *stream);
+size_t qh_ewald3_long_shifts_scratch(const qpms_ewald3_constants_t *c, size_t nK) {
+	return qh_sigma_long_shifts_scratch(c->lMax, (int)nK);
+}
+int qh_ewald3_sigma_long_shifts_dev(const qpms_ewald3_constants_t *c, double eta, double unitcell_area, size_t ntrip,
+		const double *triples_dev, size_t nK, const double *K_dev, void *F_dev, _Complex double *sigma_long_dev,
+		void *stream) {
+	qh_check_le(c->lMax);
+	const qh_e3dev *e = qh_e3_get(c);
+	return qh_launch_sigma_long_shifts(c->lMax, (int)ntrip, triples_dev, (int)nK, K_dev, eta, unitcell_area, e->leg0,
+		e->s1, e->s1off, (double *)F_dev, (double *)sigma_long_dev, stream);
+}
+
 /* ---------------------------------------------------------------------- */
 /* σ0, σ short, σ long (single triple, host arrays)                          */
 /* ---------------------------------------------------------------------- */

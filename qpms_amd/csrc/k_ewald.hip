@@ -127,6 +127,93 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
 //   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
 // s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
 // ---------------------------------------------------------------------------
+// The terms of one K point without the phase e^{i K.s}: f[e] for the
+// coefficients with n + m even, e = ev_index(n, m) (the others vanish for an
+// in-plane shift). Returns false when every Gamma(1/2-j, z) underflows.
+template <int LE> constexpr int ev_count() { return (LE + 1) * (LE + 2) / 2; }
+__host__ __device__ constexpr int ev_index(int n, int m) { return n * (n + 1) / 2 + (m + n) / 2; }
+
+template <int LE>
+__device__ __forceinline__ bool long_point(const double *__restrict__ T, double Kx, double Ky, double eta,
+                                           const double *__restrict__ leg0, const v2d *__restrict__ s1,
+                                           const int *__restrict__ s1off, cplx (&f)[ev_count<LE>()]) {
+  constexpr int JM = LE / 2;
+  const cplx k = mk(T[3], T[4]);
+  const bool kreal = (T[4] == 0.0);
+  const double bx = Kx + T[0], by = Ky + T[1];
+  const double rb = sqrt(bx * bx + by * by);
+  const double alpha = atan2(by, bx);
+  // gamma, x^2 and the incomplete-gamma branch (ewald.c:384-396)
+  cplx gam, z;
+  int branch;
+  if (kreal) {
+    const double tt = rb / k.re;
+    if (tt >= 1.0) {
+      gam = mk(sqrt(tt - 1.0) * sqrt(tt + 1.0), 0.0);
+      const double x = gam.re * k.re / (2.0 * eta);
+      z = mk(x * x, 0.0);
+    } else {
+      gam = mk(0.0, -(sqrt(1.0 - tt) * sqrt(1.0 + tt)));
+      const double x = gam.im * k.re / (2.0 * eta);
+      z = mk(-(x * x), -0.0);  // (i x)^2 lies just below the cut
+    }
+    branch = 0;
+  } else {
+    gam = qew::clilgamma(mk(rb, 0.0) / k);
+    const cplx x = (gam * k) / (2.0 * eta);
+    z = x * x;
+    branch = (z.re < 0 && !signbit(z.im)) ? -1 : 0;
+  }
+  if (z.re > 740.0) return false;  // every Gamma(1/2-j, z) underflows
+  cplx G[JM + 1];
+#pragma unroll
+  for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
+  // gamma^{2j-1}
+  cplx gp[JM + 1];
+  gp[0] = cinv(gam);
+  const cplx g2 = gam * gam;
+#pragma unroll
+  for (int j = 1; j <= JM; ++j) gp[j] = (j == 1) ? gam : gp[j - 1] * g2;
+  // (|beta+K|/k)^p, p = 0..LE
+  const cplx rk = kreal ? mk(rb / k.re, 0.0) : mk(rb, 0.0) / k;
+  cplx rkp[LE + 1];
+  rkp[0] = mk(1.0, 0.0);
+#pragma unroll
+  for (int p = 1; p <= LE; ++p) rkp[p] = rkp[p - 1] * rk;
+  const cplx e1 = cexpi(alpha);
+  cplx em[LE + 1];
+  em[0] = mk(1.0, 0.0);
+#pragma unroll
+  for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
+#pragma unroll
+  for (int n = 0; n <= LE; ++n) {
+#pragma unroll
+    for (int m = -n; m <= n; m += 2) {
+      const int am = cabsi(m);
+      const int y = n * (n + 1) + m;
+      cplx js = mk(0.0, 0.0);
+      const int off = s1off[y];
+#pragma unroll
+      for (int j = 0; j <= (n - am) / 2; ++j) {
+        const v2d c = s1[off + j];
+        js = js + ((rkp[n - 2 * j] * gp[j]) * G[j]) * mk(c.x, c.y);
+      }
+      const cplx e = (m >= 0) ? em[am] : conj(em[am]);
+      cplx t = (leg0[n * (n + 1) / 2 + am] * js) * e;
+      if (m < 0 && (am & 1)) t = -t;
+      f[ev_index(n, m)] = t;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Long range, lattice in the xy plane, in-plane shift (ewald.c:286-503):
+//   sigma^L_{n,m} = 1/(k^2 A) sum_K e^{i K.s} e^{i m alpha} P_n^{|m|}(0) (-1)^m[m<0]
+//     sum_j (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, x^2) c_{n,m,j},
+//   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
+// s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
+// ---------------------------------------------------------------------------
 template <int LE>
 __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__restrict__ tr, int npts,
                                                     const double *__restrict__ Kp /* x, y */, double eta,
@@ -134,7 +221,6 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
                                                     const v2d *__restrict__ s1, const int *__restrict__ s1off,
                                                     v2d *__restrict__ out, double *__restrict__ err) {
   constexpr int NSC = (LE + 1) * (LE + 1);
-  constexpr int JM = LE / 2;
   const int t = blockIdx.x;
   if (t >= ntrip) return;
   const int lane = threadIdx.x;
@@ -147,71 +233,16 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
   double aerr = 0.0;
   for (int ip = lane; ip < npts; ip += 64) {
     const double Kx = Kp[2 * ip], Ky = Kp[2 * ip + 1];
-    const double bx = Kx + T[0], by = Ky + T[1];
-    const double rb = sqrt(bx * bx + by * by);
-    const double alpha = atan2(by, bx);
-    // gamma, x^2 and the incomplete-gamma branch (ewald.c:384-396)
-    cplx gam, z;
-    int branch;
-    if (kreal) {
-      const double tt = rb / k.re;
-      if (tt >= 1.0) {
-        gam = mk(sqrt(tt - 1.0) * sqrt(tt + 1.0), 0.0);
-        const double x = gam.re * k.re / (2.0 * eta);
-        z = mk(x * x, 0.0);
-      } else {
-        gam = mk(0.0, -(sqrt(1.0 - tt) * sqrt(1.0 + tt)));
-        const double x = gam.im * k.re / (2.0 * eta);
-        z = mk(-(x * x), -0.0);  // (i x)^2 lies just below the cut
-      }
-      branch = 0;
-    } else {
-      gam = qew::clilgamma(mk(rb, 0.0) / k);
-      const cplx x = (gam * k) / (2.0 * eta);
-      z = x * x;
-      branch = (z.re < 0 && !signbit(z.im)) ? -1 : 0;
-    }
-    if (z.re > 740.0) continue;  // every Gamma(1/2-j, z) underflows
-    cplx G[JM + 1];
-#pragma unroll
-    for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
-    // gamma^{2j-1}
-    cplx gp[JM + 1];
-    gp[0] = cinv(gam);
-    const cplx g2 = gam * gam;
-#pragma unroll
-    for (int j = 1; j <= JM; ++j) gp[j] = (j == 1) ? gam : gp[j - 1] * g2;
-    // (|beta+K|/k)^p, p = 0..LE
-    const cplx rk = kreal ? mk(rb / k.re, 0.0) : mk(rb, 0.0) / k;
-    cplx rkp[LE + 1];
-    rkp[0] = mk(1.0, 0.0);
-#pragma unroll
-    for (int p = 1; p <= LE; ++p) rkp[p] = rkp[p - 1] * rk;
+    cplx f[ev_count<LE>()];
+    if (!long_point<LE>(T, Kx, Ky, eta, leg0, s1, s1off, f)) continue;
     const cplx phase = cexpi(Kx * T[5] + Ky * T[6]);
-    const cplx e1 = cexpi(alpha);
-    cplx em[LE + 1];
-    em[0] = mk(1.0, 0.0);
-#pragma unroll
-    for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
 #pragma unroll
     for (int n = 0; n <= LE; ++n) {
 #pragma unroll
-      for (int m = -n; m <= n; ++m) {
-        if ((n + m) & 1) continue;
-        const int am = cabsi(m);
-        const int y = n * (n + 1) + m;
-        cplx js = mk(0.0, 0.0);
-        const int off = s1off[y];
-#pragma unroll
-        for (int j = 0; j <= (n - am) / 2; ++j) {
-          const v2d c = s1[off + j];
-          js = js + ((rkp[n - 2 * j] * gp[j]) * G[j]) * mk(c.x, c.y);
-        }
-        const cplx e = (m >= 0) ? em[am] : conj(em[am]);
-        cplx s = (leg0[n * (n + 1) / 2 + am] * js) * e * phase;
-        if (m < 0 && (am & 1)) s = -s;
-        acc[y] = acc[y] + s;
-        aerr += fabs(s.re) + fabs(s.im);
+      for (int m = -n; m <= n; m += 2) {
+        const cplx v = f[ev_index(n, m)] * phase;
+        acc[n * (n + 1) + m] = acc[n * (n + 1) + m] + v;
+        aerr += fabs(v.re) + fabs(v.im);
       }
     }
   }
@@ -227,6 +258,69 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
   aerr = wave_sum(aerr);
   if (err && lane == 0)
     for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr * cabsd(cf);
+}
+
+// Long range for many shifts sharing (beta, k), as two passes: the terms of
+// every K point once (k_long_coef, the same per-point arithmetic as
+// k_sigma_long, scaled by 1/(k^2 A)), then per shift the phase sum
+// sum_K e^{i K.s} F_K (k_long_shifts: one thread per shift, K points staged
+// in LDS tiles shared by the block's shifts).
+template <int LE>
+__global__ __launch_bounds__(64) void k_long_coef(const double *__restrict__ T, int npts,
+                                                   const double *__restrict__ Kp, double eta, double area,
+                                                   const double *__restrict__ leg0, const v2d *__restrict__ s1,
+                                                   const int *__restrict__ s1off, v2d *__restrict__ F) {
+  constexpr int NE = ev_count<LE>();
+  const int ip = blockIdx.x * 64 + threadIdx.x;
+  if (ip >= npts) return;
+  const cplx k = mk(T[3], T[4]);
+  const cplx cf = (T[4] == 0.0) ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
+  cplx f[NE];
+  const bool live = long_point<LE>(T, Kp[2 * ip], Kp[2 * ip + 1], eta, leg0, s1, s1off, f);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const cplx v = live ? f[e] * cf : mk(0.0, 0.0);
+    F[(size_t)ip * NE + e] = v2d{v.re, v.im};
+  }
+}
+
+constexpr int LS_KT = 16;   // K points per LDS tile
+template <int LE>
+__global__ __launch_bounds__(256) void k_long_shifts(int ntrip, const double *__restrict__ tr, int npts,
+                                                      const double *__restrict__ Kp, const v2d *__restrict__ F,
+                                                      v2d *__restrict__ out) {
+  constexpr int NE = ev_count<LE>(), NSC = (LE + 1) * (LE + 1);
+  __shared__ v2d sF[LS_KT * NE];
+  __shared__ double sK[2 * LS_KT];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const bool live = t < ntrip;
+  const double sx = live ? tr[8 * (size_t)t + 5] : 0.0, sy = live ? tr[8 * (size_t)t + 6] : 0.0;
+  cplx acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) acc[e] = mk(0.0, 0.0);
+  for (int k0 = 0; k0 < npts; k0 += LS_KT) {
+    const int nk = min(LS_KT, npts - k0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nk * NE; i += 256) sF[i] = F[(size_t)k0 * NE + i];
+    if (threadIdx.x < 2 * nk) sK[threadIdx.x] = Kp[2 * k0 + threadIdx.x];
+    __syncthreads();
+    for (int q = 0; q < nk; ++q) {
+      const cplx ph = cexpi(sK[2 * q] * sx + sK[2 * q + 1] * sy);
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const v2d f = sF[q * NE + e];
+        acc[e] = acc[e] + ph * mk(f.x, f.y);
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int n = 0; n <= LE; ++n)
+#pragma unroll
+    for (int m = -n; m <= n; ++m) {
+      const cplx v = ((n + m) & 1) ? mk(0.0, 0.0) : acc[ev_index(n, m)];
+      out[(size_t)t * NSC + n * (n + 1) + m] = v2d{v.re, v.im};
+    }
 }
 
 // sigma0 = Gamma(-1/2, -(k/2 eta)^2) P_0^0(0) / (2 sqrt(pi)) (ewald.c:260-273)
@@ -363,6 +457,29 @@ extern "C" int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
   }
 #undef QH_SL
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Long range for ntrip triples that all share (beta, k) (those of triple 0);
+// F: scratch of npts * (LE+1)(LE+2)/2 complex values.
+extern "C" size_t qh_sigma_long_shifts_scratch(int LE, int npts) { return (size_t)npts * (LE + 1) * (LE + 2) / 2 * 16; }
+extern "C" int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev,
+                                           double eta, double area, const double *leg0_dev, const double *s1_dev,
+                                           const int *s1off_dev, double *F_dev, double *out_dev, void *stream) {
+  if (ntrip <= 0) return 0;
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  const dim3 gc((npts + 63) / 64), gs((ntrip + 255) / 256);
+#define QH_LS(L) case L: \
+    if (npts > 0) hipLaunchKernelGGL(k_long_coef<L>, gc, dim3(64), 0, s, tr_dev, npts, K_dev, eta, area, leg0_dev, \
+                                     (const v2d *)s1_dev, s1off_dev, (v2d *)F_dev); \
+    hipLaunchKernelGGL(k_long_shifts<L>, gs, dim3(256), 0, s, ntrip, tr_dev, npts, K_dev, (const v2d *)F_dev, \
+                       (v2d *)out_dev); break;
+  switch (LE) {
+    QH_LS(3) QH_LS(5) QH_LS(7) QH_LS(9)
+    default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
+  }
+#undef QH_LS
   HIPCHK(hipGetLastError());
   return 0;
 }

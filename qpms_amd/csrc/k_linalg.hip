@@ -234,6 +234,18 @@ __global__ void k_gather_rows(int n, int ncols, const int *perm, long sP, const 
   Y[(long)i * ldy + c] = X[(long)perm[i] * ldx + c];
 }
 
+// Y[i][:] = X[i][:] for n rows of ncols (batched, per-matrix strides)
+template <class T>
+__global__ void k_copy_rows(int n, int ncols, const typename C2<T>::t *X, long ldx, long sX, typename C2<T>::t *Y,
+                            long ldy, long sY) {
+  X += (long)blockIdx.z * sX;
+  Y += (long)blockIdx.z * sY;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)n * ncols) return;
+  const int i = (int)(e / ncols), c = (int)(e % ncols);
+  Y[(long)i * ldy + c] = X[(long)i * ldx + c];
+}
+
 // perm from LAPACK-style pivots (single thread per matrix; used only when a
 // caller hands in pivots without the permutation).
 __global__ void k_perm_from_ipiv(int n, const int *ipiv, int *perm) {
@@ -872,10 +884,9 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
       hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (r0 + GT - 1) / GT, batch), b256, 0, s, r0, nrhs, w,
                          LU + r0, lda, sA, (const cT *)(work + (long)r0 * nrhs), (long)nrhs, sW, work, (long)nrhs, sW);
   }
-  // copy back (B may have a different leading dimension)
-  for (int z = 0; z < batch; ++z)
-    HIPCHK(hipMemcpy2DAsync(B + (long)z * sB, ldb * sizeof(cT), work + (long)z * sW, (long)nrhs * sizeof(cT),
-                            (long)nrhs * sizeof(cT), n, hipMemcpyDeviceToDevice, s));
+  // copy back (B may have a different leading dimension), one launch for the batch
+  hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs,
+                     (const cT *)work, (long)nrhs, sW, B, ldb, sB);
   HIPCHK(hipGetLastError());
 }
 

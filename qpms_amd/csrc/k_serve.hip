@@ -49,19 +49,31 @@ __global__ void k_pad_identity(int L, int npad, v2d *__restrict__ M, long sM) {
   M[(size_t)z * sM + e] = (r == c) ? v2d{1, 0} : v2d{0, 0};
 }
 
-// out[z] = sum |x_z|^2 over n entries (one block per batch item).
-__global__ void k_sumsq_batched(long n, const v2d *__restrict__ x, long sx, double *__restrict__ out) {
-  __shared__ double red[256];
-  const v2d *xb = x + (size_t)blockIdx.x * sx;
-  double s = 0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) s += xb[i].x * xb[i].x + xb[i].y * xb[i].y;
+// sum |x_z|^2 over n entries in two deterministic passes: P partial sums
+// per batch item (grid P x batch), then one block per item adds them.
+__device__ __forceinline__ double block_sum256(double s, double *red) {
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  return red[0];
+}
+__global__ void k_sumsq_part(long n, const v2d *__restrict__ x, long sx, int P, double *__restrict__ part) {
+  __shared__ double red[256];
+  const v2d *xb = x + (size_t)blockIdx.y * sx;
+  double s = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)P * 256) s += xb[i].x * xb[i].x + xb[i].y * xb[i].y;
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * P + blockIdx.x] = s;
+}
+__global__ void k_sumsq_fin(int P, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int i = threadIdx.x; i < P; i += 256) s += part[(size_t)blockIdx.x * P + i];
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // log|det| and arg det of an LU with 1-based pivots: sum log|u_ii|,
@@ -231,9 +243,15 @@ extern "C" int qh_launch_pad_identity(int batch, int L, int npad, double *M, lon
   return 0;
 }
 
-extern "C" int qh_launch_sumsq_batched(int batch, long n, const double *x, long sx, double *out, void *stream) {
-  hipLaunchKernelGGL(k_sumsq_batched, dim3(batch), dim3(256), 0, (hipStream_t)qh_stream(stream), n, (const v2d *)x, sx,
-                     out);
+// part: batch * QH_SUMSQ_PARTS doubles of scratch.
+#define QH_SUMSQ_PARTS 512
+extern "C" int qh_launch_sumsq_batched(int batch, long n, const double *x, long sx, double *part, double *out,
+                                       void *stream) {
+  const long p = (n + 4095) / 4096;
+  const int P = (int)(p < 1 ? 1 : (p > QH_SUMSQ_PARTS ? QH_SUMSQ_PARTS : p));
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  hipLaunchKernelGGL(k_sumsq_part, dim3(P, batch), dim3(256), 0, s, n, (const v2d *)x, sx, P, part);
+  hipLaunchKernelGGL(k_sumsq_fin, dim3(batch), dim3(256), 0, s, P, (const double *)part, out);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -40,7 +40,8 @@ int qh_launch_expand_tdiag(int lMax, int nsets, const double *R, double *td, voi
 int qh_launch_tdiag_mul(int batch, int L, int nb, int npad, const double *td, long std_, const double *a,
 		long sa, double *out, long so, void *stream);
 int qh_launch_pad_identity(int batch, int L, int npad, double *M, long sM, void *stream);
-int qh_launch_sumsq_batched(int batch, long n, const double *x, long sx, double *out, void *stream);
+int qh_launch_sumsq_batched(int batch, long n, const double *x, long sx, double *part, double *out, void *stream);
+#define QH_SUMSQ_PARTS 512
 int qh_launch_logdet(int n, const double *LU, const int *ipiv, double *out, void *stream);
 int qh_launch_probe(long n, unsigned long long seed, double *V, void *stream);
 void qh_probe_host(long n, unsigned long long seed, double *V);
@@ -149,6 +150,7 @@ typedef struct qh_job {
 	double *pos;           /* owned copy */
 	size_t L, nb;
 	double t_submit;
+	double est;            /* estimated device seconds (scheduling only) */
 	struct qh_job *next;
 } qh_job;
 
@@ -189,6 +191,8 @@ struct qpms_hip_server_t {
 	int ninflight;
 	qpms_hip_response_t *ready;
 	int nready, capready;
+	qh_job **order;        /* launch-order scratch */
+	size_t order_cap;
 	unsigned long long next_id;
 	struct timespec t0;
 };
@@ -219,6 +223,7 @@ static size_t scatter_bytes(const qpms_hip_server_t *s, size_t npad, int batch, 
 	b += 2 * align256(batch * npad * 8);               /* r32, w32 */
 	b += 2 * align256(batch * npad * 4) + align256(batch * 4);
 	b += align256(qh_getrf_ws_bytes(s->o.mixed ? 32 : 64, (int)npad, batch));
+	b += align256(batch * QH_SUMSQ_PARTS * 8);         /* norm partial sums */
 	/* per job: staged inputs, Mie output, T diagonal, type indices */
 	b += batch * (align256(max_np * 24 + 128) + align256(10 * 16) + align256(48 * 16) + align256(max_np * 4));
 	b += align256(batch * 8 * 8);
@@ -230,6 +235,7 @@ static size_t periodic_bytes(size_t L, size_t np, int nsc, size_t nR, size_t nK)
 	size_t b = align256(L * L * 16) + align256(L * 8) + align256(qh_getrf_ws_bytes(64, (int)L, 1));
 	b += align256(npairs * 64) + 3 * align256(npairs * nsc * 16);
 	b += align256(nR * 24) + align256(nK * 16) + align256(np * 24) + align256(np * 4) + 4 * align256(48 * 16);
+	b += align256(nK * 55 * 16);   /* per-point long-range terms, up to (LE+1)(LE+2)/2 = 55 each */
 	return b + 65536;
 }
 
@@ -292,11 +298,17 @@ void qpms_hip_server_destroy(qpms_hip_server_t *s) {
 	free(s->pinned.b);
 	for (int l = 1; l <= 4; ++l) qpms_trans_calculator_free(s->calc[l]);
 	free(s->ready);
+	free(s->order);
 	free(s);
 }
 
 static size_t job_L(const qpms_hip_request_t *r) { return r->npart * 2 * (size_t)qpms_lMax2nelem(r->lMax); }
 
+static void periodic_counts(const qpms_hip_request_t *r, size_t *nR, size_t *nK);
+size_t qh_ewald3_long_shifts_scratch(const qpms_ewald3_constants_t *c, size_t nK);
+int qh_ewald3_sigma_long_shifts_dev(const qpms_ewald3_constants_t *c, double eta, double unitcell_area, size_t ntrip,
+		const double *triples_dev, size_t nK, const double *K_dev, void *F_dev, _Complex double *sigma_long_dev,
+		void *stream);
 static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_chunk) {
 	const size_t L = j->L;
 	switch (j->req.kind) {
@@ -305,8 +317,9 @@ static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_c
 			return scatter_bytes(s, L, 1, j->req.npart);
 		case QPMS_HIP_REQ_PERIODIC_DET: {
 			const int LE = 2 * j->req.lMax + 1;
-			/* point counts of the reference cutoffs are about pi 64^2 and pi 32^2 */
-			return periodic_bytes(L, j->req.npart, (LE + 1) * (LE + 1), 14000, 3600);
+			size_t nR, nK;
+			periodic_counts(&j->req, &nR, &nK);
+			return periodic_bytes(L, j->req.npart, (LE + 1) * (LE + 1), nR, nK);
 		}
 		case QPMS_HIP_REQ_BEYN: {
 			const size_t l = (size_t)j->req.beyn_l;
@@ -316,6 +329,26 @@ static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_c
 			if (beyn_chunk) *beyn_chunk = c;
 			return beyn_bytes(s, L, l, c, N, lm);
 		}
+	}
+	return 0;
+}
+
+/* Rough device seconds of a job, for the launch order only: LU flops at
+ * 50 (FP32) / 35 (FP64) TFLOP/s, the refinement or node solves, the
+ * periodic lattice sums per particle pair, and a fixed launch cost. */
+static double job_est(const qpms_hip_server_t *s, const qh_job *j) {
+	const double L = (double)j->L, lu = 8.0 / 3.0 * L * L * L / (s->o.mixed ? 5e13 : 3.5e13);
+	const double np = (double)j->req.npart;
+	switch (j->req.kind) {
+		case QPMS_HIP_REQ_SCATTER: return 2e-3 + lu + 8.0 * L * L * s->o.refine_steps / 5e12;
+		case QPMS_HIP_REQ_PERIODIC_DET: {
+			const double ne = (2.0 * j->req.lMax + 1) * (2.0 * j->req.lMax + 1);
+			size_t nR, nK;
+			periodic_counts(&j->req, &nR, &nK);
+			return 5e-3 + 8.0 / 3.0 * L * L * L / 3.5e13 + np * np * (double)(nR + nK) * ne * 2e-12;
+		}
+		case QPMS_HIP_REQ_BEYN:
+			return 5e-3 + j->req.beyn_nodes * (lu + 16.0 * L * L * j->req.beyn_l * s->o.refine_steps / 5e13);
 	}
 	return 0;
 }
@@ -338,6 +371,7 @@ unsigned long long qpms_hip_server_submit(qpms_hip_server_t *s, const qpms_hip_r
 	j->L = job_L(r);
 	j->nb = 2 * (size_t)qpms_lMax2nelem(r->lMax);
 	j->t_submit = qpms_hip_server_clock(s);
+	j->est = job_est(s, j);
 	if (((job_bytes(s, j, NULL) + 4095) & ~(size_t)4095) > s->arena.size) { free(j->pos); free(j); return 0; }
 	j->id = s->next_id++;
 	if (s->qtail) s->qtail->next = j; else s->qhead = j;
@@ -402,6 +436,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	int *ipiv = bump(&bp, B * npad * 4), *perm = bump(&bp, B * npad * 4), *info = bump(&bp, B * 4);
 	void *ws = bump(&bp, qh_getrf_ws_bytes(s->o.mixed ? 32 : 64, (int)npad, B));
 	double *outs = bump(&bp, B * 8 * 8);   /* per job: dot, |r|^2, |x|^2, |M|^2, info */
+	double *part = bump(&bp, B * QH_SUMSQ_PARTS * 8);   /* partial sums of the norms */
 	/* staged inputs: per job pos | mie params | E (6 doubles) */
 	size_t in_off[MAX_BATCH], in_bytes = 0;
 	for (int j = 0; j < B; ++j) { in_off[j] = in_bytes; in_bytes += align256(u->jobs[j]->req.npart * 24 + 9 * 8 + 6 * 8); }
@@ -440,7 +475,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 		qh_launch_tdiag_mul(1, (int)L, (int)nb, (int)npad, td, 0, a + j * 2 * npad, 0, b + j * 2 * npad, 0, st);
 	}
 	const long sM = (long)nn, sV = (long)npad;
-	qh_launch_sumsq_batched(B, (long)nn, M, sM, outs + 3 * B, st);
+	qh_launch_sumsq_batched(B, (long)nn, M, sM, part, outs + 3 * B, st);
 	if (s->o.mixed) {
 		qh_d2f((long)(B * nn), M, M32, st);
 		qh_getrf_batched_ws_f32((int)npad, B, M32, sM, ipiv, perm, info, ws, st);
@@ -460,8 +495,8 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 		qh_getrs_batched_f64((int)npad, 1, B, M, sM, perm, x, sV, work, st);
 		SRV_HIPCHK(hipMemsetAsync(r, 0, B * npad * 16, st));
 	}
-	qh_launch_sumsq_batched(B, (long)npad, r, sV, outs + B, st);
-	qh_launch_sumsq_batched(B, (long)npad, x, sV, outs + 2 * B, st);
+	qh_launch_sumsq_batched(B, (long)npad, r, sV, part, outs + B, st);
+	qh_launch_sumsq_batched(B, (long)npad, x, sV, part, outs + 2 * B, st);
 	qh_launch_neg_re_cdot(B, (long)npad, a, sV, x, sV, outs, st);
 	SRV_HIPCHK(hipMemcpyAsync(outs + 4 * B, info, B * 4, hipMemcpyDeviceToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(u->pin + in_bytes, outs, B * 8 * 8, hipMemcpyDeviceToHost, st));
@@ -489,21 +524,21 @@ static void finish_scatter(qpms_hip_server_t *s, qh_unit *u, qpms_hip_response_t
 typedef struct { double *R, *K; size_t nR, nK; } qh_points;
 
 /* Reference cutoffs of the periodic builders (scatsystem.c:1226-1227). */
-static void periodic_points(const qpms_hip_request_t *r, qh_points *pt, double *area, double *eta) {
-	cart2_t b1 = {r->lattice[0][0], r->lattice[0][1]}, b2 = {r->lattice[1][0], r->lattice[1][1]};
-	l2d_reduceBasis(b1, b2, &b1, &b2);
-	*area = l2d_unitcell_area(b1, b2);
-	const double maxR = sqrt(*area) * 64, maxK = 2048 * 2 * M_PI / maxR;
-	cart2_t rb1, rb2, z = {0, 0};
-	QH_ENSURE(l2d_reciprocalBasis2pi(b1, b2, &rb1, &rb2) == 0, "degenerate lattice");
-	PGen g = PGen_xyWeb_new(b1, b2, 1e-13, z, 0, true, maxR, false);
-	pt->R = qh_collect(&g, &pt->nR, 3);
-	for (size_t i = 0; i < pt->nR; ++i) pt->R[3 * i + 2] = 0;
-	PGen_destroy(&g);
-	g = PGen_xyWeb_new(rb1, rb2, 1e-13, z, 0, true, maxK, false);
-	pt->K = qh_collect(&g, &pt->nK, 2);
-	PGen_destroy(&g);
-	/* adjusted eta (scatsystem.c:68-80) with eta_default = sqrt(pi / A) */
+/* Ewald parameters of a periodic request: eta is the reference's adjusted
+ * value (scatsystem.c:68-80, default sqrt(pi / A)); the reciprocal cutoff
+ * is the reference's, maxK = 2048 2 pi / (64 sqrt(A)) (its truncation is
+ * part of the value: with a raised eta the K sum is not converged there).
+ * The direct-space cutoff is the reference's 64 sqrt(A) or, if smaller, the
+ * radius beyond which every term has |R - s|^2 eta^2 - k^2 / (4 eta^2) >=
+ * EW_X: such terms carry e^{-EW_X} = 1.6e-28 times polynomial prefactors of
+ * at most (2 |R - s| / k)^9, far below FP64 rounding of the retained sum,
+ * where the reference sums them down to underflow. */
+#define EW_X 64.0
+static void periodic_params(const qpms_hip_request_t *r, double *area, double *eta, double *maxR, double *maxK,
+		cart2_t *b1, cart2_t *b2) {
+	cart2_t a1 = {r->lattice[0][0], r->lattice[0][1]}, a2 = {r->lattice[1][0], r->lattice[1][1]};
+	l2d_reduceBasis(a1, a2, b1, b2);
+	*area = l2d_unitcell_area(*b1, *b2);
 	const double eta0 = 2. / M_2_SQRTPI / sqrt(*area);
 	const double k = wavenumber_re(r->eps_bg, r->omega);
 	const double k2 = r->beta[0] * r->beta[0] + r->beta[1] * r->beta[1], kappa2 = k * k;
@@ -513,6 +548,40 @@ static void periodic_points(const qpms_hip_request_t *r, qh_points *pt, double *
 		const double eta_min = sqrt(fabs((kappa2 - k2) * (mina - 1.) / 16.));   /* scatsystem.c:42 */
 		*eta = fmax(eta0, eta_min);
 	}
+	/* largest in-cell shift |r_i - r_j|: the bounding-box diagonal */
+	double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+	for (size_t i = 0; i < r->npart; ++i)
+		for (int c = 0; c < 2; ++c) { lo[c] = fmin(lo[c], r->pos[3 * i + c]); hi[c] = fmax(hi[c], r->pos[3 * i + c]); }
+	const double diam = hypot(hi[0] - lo[0], hi[1] - lo[1]);
+	const double e = *eta, refR = 64 * sqrt(*area);
+	*maxR = fmin(refR, sqrt(EW_X + kappa2 / (4 * e * e)) / e + diam);
+	*maxK = 2048 * 2 * M_PI / refR;
+}
+
+static void periodic_points(const qpms_hip_request_t *r, qh_points *pt, double *area, double *eta) {
+	cart2_t b1, b2, rb1, rb2, z = {0, 0};
+	double maxR, maxK;
+	periodic_params(r, area, eta, &maxR, &maxK, &b1, &b2);
+	QH_ENSURE(l2d_reciprocalBasis2pi(b1, b2, &rb1, &rb2) == 0, "degenerate lattice");
+	PGen g = PGen_xyWeb_new(b1, b2, 1e-13, z, 0, true, maxR, false);
+	pt->R = qh_collect(&g, &pt->nR, 3);
+	for (size_t i = 0; i < pt->nR; ++i) pt->R[3 * i + 2] = 0;
+	PGen_destroy(&g);
+	g = PGen_xyWeb_new(rb1, rb2, 1e-13, z, 0, true, maxK, false);
+	pt->K = qh_collect(&g, &pt->nK, 2);
+	PGen_destroy(&g);
+}
+
+/* Point counts of periodic_points (the generator's layers are polygons
+ * around the disc, so the points are counted, not estimated). */
+static void periodic_counts(const qpms_hip_request_t *r, size_t *nR, size_t *nK) {
+	qh_points pt;
+	double area, eta;
+	periodic_points(r, &pt, &area, &eta);
+	*nR = pt.nR;
+	*nK = pt.nK;
+	free(pt.R);
+	free(pt.K);
 }
 
 static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
@@ -523,13 +592,17 @@ static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	qh_points pt;
 	double area, eta;
 	periodic_points(r, &pt, &area, &eta);
-	qh_bump bp = {s->arena.base + u->arena_off, periodic_bytes(L, np, nsc, pt.nR, pt.nK)};
+	size_t cR, cK;
+	periodic_counts(r, &cR, &cK);
+	QH_ENSURE(pt.nR == cR && pt.nK == cK, "lattice point counts differ between admission and launch");
+	qh_bump bp = {s->arena.base + u->arena_off, periodic_bytes(L, np, nsc, cR, cK)};
 	double *M = bump(&bp, L * L * 16);
 	int *ipiv = bump(&bp, L * 4), *perm = bump(&bp, L * 4);
 	void *ws = bump(&bp, qh_getrf_ws_bytes(64, (int)L, 1));
 	double *tr = bump(&bp, np * np * 64);
 	double *ss = bump(&bp, np * np * nsc * 16), *sl = bump(&bp, np * np * nsc * 16), *tot = bump(&bp, np * np * nsc * 16);
 	double *dR = bump(&bp, pt.nR * 24), *dK = bump(&bp, pt.nK * 16), *dpos = bump(&bp, np * 24);
+	void *lsF = bump(&bp, qh_ewald3_long_shifts_scratch(srv_calc(s, lMax)->e3c, pt.nK));
 	int *tt = bump(&bp, np * 4);
 	double *mie = bump(&bp, 2 * (lMax + 1) * 16), *td = bump(&bp, nb * 16), *dk = bump(&bp, 64), *s0 = bump(&bp, 64);
 	double *outs = bump(&bp, 64), *dprm = bump(&bp, 128);
@@ -555,8 +628,8 @@ static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	SRV_HIPCHK(hipMemcpyAsync(dk, dprm + 9, 16, hipMemcpyDeviceToDevice, st));
 	const qpms_trans_calculator *c = srv_calc(s, lMax);
 	qh_launch_pair_triples((int)np, dpos, r->beta[0], r->beta[1], k, 0.0, tr, st);
-	qpms_hip_ewald3_sigma_batch_dev(c->e3c, eta, area, np * np, tr, pt.nR, dR, pt.nK, dK,
-		(complex double *)ss, (complex double *)sl, st);
+	qpms_hip_ewald3_sigma_batch_dev(c->e3c, eta, area, np * np, tr, pt.nR, dR, 0, NULL, (complex double *)ss, NULL, st);
+	qh_ewald3_sigma_long_shifts_dev(c->e3c, eta, area, np * np, tr, pt.nK, dK, lsF, (complex double *)sl, st);
 	qh_launch_sigma0(1, dk, eta, c->e3c->legendre0[0], s0, st);
 	qh_launch_pair_sigma_total((int)np, nsc, ss, sl, s0, tot, st);
 	qh_launch_e32_assemble_M(qh_tables_get(c), (int)np, nsc, tot, qh_e32_bq_mode(), tt, td, M, (long)L, st);
@@ -683,7 +756,7 @@ static void finish_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, qpms_h
 		snprintf(out->message, sizeof out->message, "cancelled after %d of %d nodes", u->next_node, r->beyn_nodes);
 		return;
 	}
-	const int K = qh_beyn_process_s(jb->L, l, u->dA0, u->dA1, CMPLX(r->omega, r->beyn_centre_im), r->rank_tol, 1,
+	const int K = qh_beyn_process_s(jb->L, l, u->dA0, u->dA1, CMPLX(r->omega, r->beyn_centre_im), r->rank_tol, 0,
 		out->eigval, out->sv, st,
 		u->btail);
 	out->neig = K;
@@ -735,28 +808,54 @@ static void retire(qpms_hip_server_t *s, qh_unit *u) {
 	--s->ninflight;
 }
 
+/* Launch order: shortest estimated work first, with aging (a job's key
+ * drops by SRV_AGING seconds per second waited), so that short requests
+ * are not held behind long contours while long ones still start
+ * eventually. Ties keep arrival order. */
+#define SRV_AGING 0.1
+static double g_key_now;
+static int key_cmp(const void *a, const void *b) {
+	const qh_job *x = *(qh_job *const *)a, *y = *(qh_job *const *)b;
+	const double kx = x->est - SRV_AGING * (g_key_now - x->t_submit), ky = y->est - SRV_AGING * (g_key_now - y->t_submit);
+	if (kx != ky) return kx < ky ? -1 : 1;
+	return x->id < y->id ? -1 : (x->id > y->id);
+}
+
+static qh_job *q_prev(qpms_hip_server_t *s, qh_job *j) {
+	qh_job *pv = NULL;
+	for (qh_job *c = s->qhead; c && c != j; c = c->next) pv = c;
+	return pv;
+}
+
 static int try_launch(qpms_hip_server_t *s) {
 	const int lane = free_lane(s);
 	if (lane < 0 || !s->qhead) return 0;
-	/* first queued job that fits (arrival order, backfilling past jobs that do not) */
-	qh_job *prev = NULL;
-	for (qh_job *j = s->qhead; j; prev = j, j = j->next) {
+	if (s->nqueued > s->order_cap) {
+		s->order_cap = s->nqueued * 2;
+		s->order = realloc(s->order, s->order_cap * sizeof(*s->order));
+		QH_ENSURE(s->order, "oom");
+	}
+	size_t nq = 0;
+	for (qh_job *c = s->qhead; c; c = c->next) s->order[nq++] = c;
+	g_key_now = qpms_hip_server_clock(s);
+	qsort(s->order, nq, sizeof(*s->order), key_cmp);
+	/* first job in that order that fits (backfilling past jobs that do not) */
+	for (size_t oi = 0; oi < nq; ++oi) {
+		qh_job *j = s->order[oi];
 		int chunk = 1;
 		const int small = j->req.kind == QPMS_HIP_REQ_SCATTER && j->L <= s->o.batch_max_dim;
 		qh_unit *u = unit_new(s);
 		if (!u) return 0;
 		if (small) {
-			/* batch: this job and later queued small solves of the same padded order */
+			/* batch: this job and the next queued small solves (launch order) of the same padded order */
 			const size_t npad = pad_bucket(j->L);
-			qh_job *members[MAX_BATCH], *mprev[MAX_BATCH];
+			qh_job *members[MAX_BATCH];
 			int B = 0;
-			size_t max_np = 0;
-			qh_job *pp = prev;
-			for (qh_job *c = j; c && B < s->o.batch_max; pp = c, c = c->next)
-				if (c->req.kind == QPMS_HIP_REQ_SCATTER && c->L <= s->o.batch_max_dim && pad_bucket(c->L) == npad) {
-					members[B] = c; mprev[B] = pp; ++B;
-					if (c->req.npart > max_np) max_np = c->req.npart;
-				}
+			for (size_t q = oi; q < nq && B < s->o.batch_max; ++q) {
+				qh_job *c = s->order[q];
+				if (c->req.kind == QPMS_HIP_REQ_SCATTER && c->L <= s->o.batch_max_dim && pad_bucket(c->L) == npad)
+					members[B++] = c;
+			}
 			long off = -1;
 			while (B > 0) {
 				size_t mnp = 0;
@@ -766,13 +865,7 @@ static int try_launch(qpms_hip_server_t *s) {
 				--B;
 			}
 			if (off < 0) { u->active = 0; continue; }
-			/* unlink members (in reverse so the recorded predecessors stay valid) */
-			for (int q = B - 1; q >= 0; --q) {
-				qh_job *pv = NULL;
-				for (qh_job *c = s->qhead; c && c != members[q]; c = c->next) pv = c;
-				q_remove(s, pv, members[q]);
-			}
-			(void)mprev;
+			for (int q = 0; q < B; ++q) q_remove(s, q_prev(s, members[q]), members[q]);
 			u->kind = U_SCATTER;
 			u->njobs = B;
 			for (int q = 0; q < B; ++q) u->jobs[q] = members[q];
@@ -782,7 +875,7 @@ static int try_launch(qpms_hip_server_t *s) {
 			const size_t need = job_bytes(s, j, &chunk);
 			const long off = arena_alloc(&s->arena, need);
 			if (off < 0) { u->active = 0; continue; }
-			q_remove(s, prev, j);
+			q_remove(s, q_prev(s, j), j);
 			u->kind = (j->req.kind == QPMS_HIP_REQ_SCATTER) ? U_SCATTER : (j->req.kind == QPMS_HIP_REQ_PERIODIC_DET) ? U_PERIODIC : U_BEYN;
 			u->njobs = 1;
 			u->jobs[0] = j;
@@ -820,11 +913,10 @@ int qpms_hip_server_progress(qpms_hip_server_t *s, int wait) {
 		}
 		while (try_launch(s)) {}
 		if (!wait || s->nready > 0 || s->ninflight == 0) return s->nready;
-		/* block on the oldest in-flight unit */
-		qh_unit *oldest = NULL;
-		for (int i = 0; i < MAX_UNITS; ++i)
-			if (s->unit[i].active && s->unit[i].done && (!oldest || s->unit[i].t_start < oldest->t_start)) oldest = &s->unit[i];
-		if (oldest) SRV_HIPCHK(hipEventSynchronize(oldest->done));
+		/* wait for whichever unit finishes first: poll every event (a blocking
+		 * wait on one of them would hold up the others' retirement) */
+		const struct timespec nap = {0, 50000};
+		nanosleep(&nap, NULL);
 	}
 }
 

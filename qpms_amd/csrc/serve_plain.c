@@ -127,7 +127,7 @@ int qpms_hip_request_plain(const qpms_hip_request_t *r, qpms_hip_response_t *out
 		beyn_contour_t *c = beyn_contour_ellipse(CMPLX(r->omega, r->beyn_centre_im), r->beyn_rRe, r->beyn_rIm,
 			(size_t)N);
 		struct qh_plain_beyn p = {ss};
-		beyn_result_t *br = beyn_solve_v(L, l, plain_beyn_M, NULL, &p, c, rank_tol, 1, 0.0, V);
+		beyn_result_t *br = beyn_solve_v(L, l, plain_beyn_M, NULL, &p, c, rank_tol, 0, 0.0, V);
 		out->neig = (int)br->neig;
 		for (size_t i = 0; i < br->neig && i < QPMS_HIP_SERVE_MAX_EIG; ++i) out->eigval[i] = br->eigval[i];
 		for (size_t i = 0; i < l && i < QPMS_HIP_SERVE_MAX_EIG; ++i) out->sv[i] = br->ranktest_SV[i];

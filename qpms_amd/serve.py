@@ -174,6 +174,7 @@ def beyn_rel_diff(a, b, rank_tol):
     keep = sb > rank_tol
     if keep.sum() != (sa > rank_tol).sum() or a.neig != b.neig:
         return float("inf")
+    # below the cut both are rounding noise of an (almost) empty contour
     err = float(np.max(np.abs(sa[keep] - sb[keep])) / sb[0]) if keep.any() else 0.0
     ea, eb = list(a.eigenvalues()), list(b.eigenvalues())
     for z in eb:

@@ -42,6 +42,13 @@ def requests():
         out.append(sv.Req(sv.KIND_BEYN, pos, lMax, 70e-9, -12 + 1j, 1.52 ** 2, c * EV, beyn_rRe=rre * EV,
                           beyn_rIm=rim * EV, beyn_nodes=16, beyn_l=16, beyn_seed=7 + N,
                           rank_tol=sv.beyn_rank_tol(N * 2 * lMax * (lMax + 2), 16, rre * EV)))
+    # larger unit cells (the served lattice sums use exponent-bounded cutoffs,
+    # the plain path the reference's)
+    rng2 = np.random.default_rng(12)
+    for N, lMax in [(16, 2), (30, 3)]:
+        pos, a = grid(N, 400e-9, rng2)
+        out.append(sv.Req(sv.KIND_PERIODIC, pos, lMax, 60e-9, -15 + 1.5j, 1.52 ** 2, 1.9 * EV,
+                          lattice=((a, 0), (0, a)), beta=tuple(rng2.uniform(-np.pi / a, np.pi / a, 2))))
     return out
 
 
@@ -115,7 +122,7 @@ def test_served_beyn_matches_oracle(gpu):
         lib.qh_probe_host(L * l, c.beyn_seed, V.ctypes.data)
         centre = complex(c.omega, c.beyn_centre_im)
         z, dz = ob.contour_ellipse(centre, c.beyn_rRe, c.beyn_rIm, c.beyn_nodes)
-        ev, _, _, S, _ = ob.beyn_solve(Mf, L, V, z, dz, centre, rank_tol=c.rank_tol, rank_sel_min=1)
+        ev, _, _, S, _ = ob.beyn_solve(Mf, L, V, z, dz, centre, rank_tol=c.rank_tol, rank_sel_min=0)
         g = got[i]
         assert g.status == 0 and g.neig == len(ev) and g.neig >= 2
         keep = S > c.rank_tol
