@@ -206,9 +206,11 @@ double qpms_hip_server_clock(const qpms_hip_server_t *s) {
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 /* Dimension buckets of the batched path (order of the padded matrices). */
+/* Padded orders of batched solves. Coarse on purpose: up to a few thousand
+ * the LU is bound by its chain of small launches, not by flops, so padding
+ * costs little and a coarser bucket gathers a larger batch. */
 static size_t pad_bucket(size_t L) {
-	static const size_t b[] = {128, 192, 256, 320, 384, 448, 512, 640, 768, 896, 1024, 1280, 1536, 1792, 2048,
-		2560, 3072, 3584, 4096};
+	static const size_t b[] = {256, 512, 1024, 1536, 2048, 3072, 4096, 6144, 8192};
 	for (size_t i = 0; i < sizeof b / sizeof b[0]; ++i) if (L <= b[i]) return b[i];
 	return (L + 255) & ~(size_t)255;
 }
@@ -266,7 +268,7 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 	if (!s->o.arena_bytes) s->o.arena_bytes = (size_t)64 << 30;
 	if (s->o.nstreams <= 0) s->o.nstreams = 4;
 	if (s->o.nstreams > MAX_LANES) s->o.nstreams = MAX_LANES;
-	if (!s->o.batch_max_dim) s->o.batch_max_dim = 2048;
+	if (!s->o.batch_max_dim) s->o.batch_max_dim = 8192;
 	if (s->o.batch_max <= 0) s->o.batch_max = 32;
 	if (s->o.batch_max > MAX_BATCH) s->o.batch_max = MAX_BATCH;
 	if (s->o.refine_steps <= 0) s->o.refine_steps = 3;
