@@ -344,6 +344,42 @@ def bench_beyn(L, window=None):
                                                                                plain.eigenvalues()]}
 
 
+def per_kind(n):
+    """Each of the first n stream requests (order < 25000, contours < 6000)
+    served alone, then through the plain path: times and differences."""
+    from qpms_amd import serve as sv
+    reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
+            if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
+    srv = sv.Server(arena_gib=160, nstreams=4, mixed=True)
+    for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
+        srv.submit(r)
+    while srv.pending():
+        srv.progress(wait=True)
+        srv.poll()
+    tot = {k: [0.0, 0.0, 0.0, 0] for k in (0, 1, 2)}
+    for i, r in enumerate(reqs):
+        t0 = time.perf_counter()
+        srv.submit(r)
+        g = None
+        while srv.pending():
+            srv.progress(wait=True)
+            for x in srv.poll():
+                g = x
+        ts = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p = sv.request_plain(r)
+        tp = time.perf_counter() - t0
+        err = rel_diff(g, p, r)
+        a = tot[r.kind]
+        a[0] += ts; a[1] += tp; a[2] = max(a[2], err); a[3] += 1
+        log(f"{i:3d} kind {r.kind} order {r.dim:6d}: served {ts:.3f} s, plain {tp:.3f} s, rel diff {err:.1e}")
+    srv.close()
+    return {"metric": "per-kind served-alone vs plain FP64 time", "requests": n,
+            "by_kind": {str(k): {"requests": a[3], "served_s": a[0], "plain_s": a[1],
+                                 "ratio": a[1] / a[0] if a[0] > 0 else None, "max_rel_diff": a[2]}
+                        for k, a in tot.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["serving", "ewald", "beyn"], default="serving")
@@ -359,6 +395,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-wbuild", action="store_true")
     ap.add_argument("--lu-probe", action="store_true", help="only the FP32 LU measurement (for rocprofv3)")
+    ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); requests would be "
@@ -375,6 +412,9 @@ def main():
 
     if args.lu_probe:
         print(json.dumps(measure_lu(L, ev, stream, reps=2)), flush=True)
+        return
+    if args.per_kind:
+        print(json.dumps(per_kind(args.per_kind)), flush=True)
         return
     if args.workload == "ewald":
         print(json.dumps(bench_ewald(L, ev, stream)), flush=True)
