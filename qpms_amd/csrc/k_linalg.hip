@@ -342,13 +342,16 @@ template <class T> constexpr int ts_fan() { return TsRows<T>::R / PW_; }   // ca
 template <class T> constexpr size_t ts_lds() { return (size_t)TsRows<T>::R * TsStride<T>::XS * sizeof(typename C2<T>::t); }
 
 // Per matrix: two candidate buffers of Gmax sets (index + PW values each),
-// then the last level's output: L11\U11, the moved-row list and its length.
+// then one output slot per strip of an outer block (the last tournament
+// level's L11\U11, moved-row list and its length), kept until the block's
+// interchanges have been applied to the columns outside it.
+constexpr int TS_SLOTS = 8;   // strips per outer block (LU_NB / PW)
 template <class T>
 struct PanelWS {
   static size_t set_bytes() { return ((size_t)PW_ * 4 + (size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 255) / 256 * 256; }
   static size_t fin_bytes() { return ((size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 4 * PW_ * 4 + 16 + 255) / 256 * 256; }
   static int sets(int n) { return (n + TsRows<T>::R - 1) / TsRows<T>::R; }
-  static size_t bytes(int n) { return (2 * (size_t)sets(n) * set_bytes() + fin_bytes() + 255) / 256 * 256; }
+  static size_t bytes(int n) { return (2 * (size_t)sets(n) * set_bytes() + TS_SLOTS * fin_bytes() + 255) / 256 * 256; }
 };
 
 template <class T>
@@ -630,35 +633,64 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin,
   }
 }
 
-// Row interchanges of one strip on every column of the matrix, as the
-// gather A[c0 + to] = A[c0 + from] over the moved rows (staged in LDS per
-// block of 64 columns), then L11\U11 into the top block.
+// Row interchanges of strips as gathers A[c0 + to] = A[c0 + from] over their
+// moved rows, staged in LDS per block of AP_COLS columns.
+//   k_ts_apply:  one strip (its slot `fin`), columns [col_lo, col_hi), then
+//                L11\U11 into the strip's top block;
+//   k_ts_apply_defer: the strips of an outer block (slots 0..nslots-1, strip
+//                c0 = K0 + PW slot, in order) on every column outside
+//                [skip_lo, skip_hi).
 constexpr int AP_COLS = 32;
 template <class T>
-__global__ __launch_bounds__(256) void k_ts_apply(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
-                                                  const char *fin, long sW) {
+__device__ __forceinline__ void ts_gather(int n, int c0, const char *fin, typename C2<T>::t *A, long lda, int j0,
+                                          typename C2<T>::t *buf, int *from, int *to, bool colok) {
   typedef typename C2<T>::t cT;
-  __shared__ cT buf[2 * PW_ * AP_COLS];
-  __shared__ int from[2 * PW_], to[2 * PW_];
-  const int z = blockIdx.z, t = threadIdx.x, j0 = blockIdx.x * AP_COLS;
-  A += (long)z * sA;
-  fin += (long)z * sW;
-  const cT *lu = reinterpret_cast<const cT *>(fin);
+  const int t = threadIdx.x;
   const int *lfrom = reinterpret_cast<const int *>(fin + PW_ * PW_ * sizeof(cT));
   const int *lto = lfrom + 2 * PW_;
   const int k = lto[2 * PW_];
   if (t < k) { from[t] = lfrom[t]; to[t] = lto[t]; }
   __syncthreads();
   const int c = t % AP_COLS, r4 = t / AP_COLS;
-  const bool colok = j0 + c < n;
   for (int e = r4; e < k; e += 256 / AP_COLS)
     if (colok) buf[e * AP_COLS + c] = A[(long)(c0 + from[e]) * lda + j0 + c];
   __syncthreads();
   for (int e = r4; e < k; e += 256 / AP_COLS)
     if (colok) A[(long)(c0 + to[e]) * lda + j0 + c] = buf[e * AP_COLS + c];
   __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_ts_apply(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
+                                                  const char *fin, long sW, int col_lo, int col_hi) {
+  typedef typename C2<T>::t cT;
+  __shared__ cT buf[2 * PW_ * AP_COLS];
+  __shared__ int from[2 * PW_], to[2 * PW_];
+  const int z = blockIdx.z, t = threadIdx.x, j0 = col_lo + blockIdx.x * AP_COLS;
+  A += (long)z * sA;
+  fin += (long)z * sW;
+  const int c = t % AP_COLS, r4 = t / AP_COLS;
+  const bool colok = j0 + c < col_hi;
+  ts_gather<T>(n, c0, fin, A, lda, j0, buf, from, to, colok);
+  const cT *lu = reinterpret_cast<const cT *>(fin);
   if (colok && j0 + c >= c0 && j0 + c < c0 + w)
     for (int i = r4; i < w; i += 256 / AP_COLS) A[(long)(c0 + i) * lda + j0 + c] = lu[i * PW_ + (j0 + c - c0)];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_ts_apply_defer(int n, int K0, int nslots, typename C2<T>::t *A, long lda,
+                                                        long sA, const char *fin0, long finb, long sW, int skip_lo,
+                                                        int skip_hi) {
+  typedef typename C2<T>::t cT;
+  __shared__ cT buf[2 * PW_ * AP_COLS];
+  __shared__ int from[2 * PW_], to[2 * PW_];
+  const int z = blockIdx.z, j0 = blockIdx.x * AP_COLS;
+  if (j0 >= skip_lo && j0 + AP_COLS <= skip_hi) return;   // whole block inside the skipped columns (uniform)
+  A += (long)z * sA;
+  fin0 += (long)z * sW;
+  const int c = threadIdx.x % AP_COLS;
+  const bool colok = j0 + c < n && (j0 + c < skip_lo || j0 + c >= skip_hi);
+  for (int q = 0; q < nslots; ++q) ts_gather<T>(n, K0 + PW_ * q, fin0 + q * finb, A, lda, j0, buf, from, to, colok);
 }
 
 // L21 = A21 U11^{-1}: one thread per strip row below the top block.
@@ -753,6 +785,37 @@ static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::
 }
 
 constexpr int LU_NB = 256;   // outer block: K of the trailing GEMM
+static_assert(LU_NB / PW_ == TS_SLOTS, "one output slot per strip of an outer block");
+
+// Lookahead context of a caller stream: the side stream that factors the
+// next block's strips and the two events ordering it against the caller's
+// stream (default priority; created once per caller stream).
+struct LaCtx {
+  hipStream_t key, side;
+  hipEvent_t ready, panel;
+};
+static LaCtx *la_for(hipStream_t s) {
+  static std::mutex mu;
+  static LaCtx tab[16];
+  static int n = 0;
+  std::lock_guard<std::mutex> g(mu);
+  for (int i = 0; i < n; ++i) if (tab[i].key == s) return &tab[i];
+  if (n == 16) qpms_hip_die(__FILE__, __LINE__, __func__, "too many streams for the LU lookahead table");
+  LaCtx &c = tab[n];
+  c.key = s;
+  HIPCHK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c.ready, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c.panel, hipEventDisableTiming));
+  return &tab[n++];
+}
+static bool use_lookahead() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("QPMS_HIP_LU_LOOKAHEAD");
+    v = !(e && !strcmp(e, "0"));
+  }
+  return v == 1;
+}
 
 // Workspace bytes for getrf_batched<T>(n, batch).
 template <class T>
@@ -789,8 +852,6 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     }
     ws = own;
   }
-  char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
-  char *fin = ws + 2 * (long)Gmax * setb;            // last level's output
   const dim3 bts(TS_THREADS);
   static bool lds_set = false;   // the tournament kernels stage a chunk of more than 64 KB
   if (!lds_set) {
@@ -798,40 +859,63 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
     lds_set = true;
   }
-  if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
-  for (int k0 = 0; k0 < n; k0 += LU_NB) {
-    const int nb = min(LU_NB, n - k0);
+  char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
+  char *fin0 = ws + 2 * (long)Gmax * setb;           // per-strip slots of the last level's output
+  const long finb = (long)PanelWS<T>::fin_bytes();
+  // The strips of the outer block [k0, k0 + nb): tournament, interchanges on
+  // the block's own columns, L21, then the block's remaining columns by TRSM
+  // and the rank-w update. Touches only the block's columns (rows >= k0).
+  auto block_strips = [&](int k0, int nb, hipStream_t st) {
+    constexpr int F = ts_fan<T>();
     for (int j0 = 0; j0 < nb; j0 += PW) {
       const int w = min(PW, nb - j0), c0 = k0 + j0;
-      // tournament-pivoted panel of the strip rows c0..n-1
-      constexpr int F = ts_fan<T>();
+      char *fin = fin0 + (long)(j0 / PW) * finb;
       const int G = (n - c0 + TsRows<T>::R - 1) / TsRows<T>::R;
-      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), s, n, c0, w, A, lda, sA, ws, sW, setb,
+      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), st, n, c0, w, A, lda, sA, ws, sW, setb,
                          (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
       char *cur = bufA, *nxt = bufB;
       for (int nsets = G; nsets > 1; nsets = (nsets + F - 1) / F) {
         const int nout = (nsets + F - 1) / F;
-        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), s, c0, w, nsets, cur, nxt, sW, setb,
-                           fin, nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
+        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), st, c0, w, nsets, cur, nxt, sW,
+                           setb, fin, nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
         char *tmp = cur; cur = nxt; nxt = tmp;
       }
-      // the interchanges on every column, L11\U11 into place, then L21
-      hipLaunchKernelGGL(k_ts_apply<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, c0, w, A, lda, sA,
-                         fin, sW);
+      hipLaunchKernelGGL(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, st, n, c0, w, A, lda,
+                         sA, fin, sW, k0, k0 + nb);
       if (n - c0 - w > 0)
-        hipLaunchKernelGGL(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, s, n, c0, w, A, lda, sA);
-      // rest of this outer block: U part by TRSM, then the rank-w update
+        hipLaunchKernelGGL(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, st, n, c0, w, A, lda, sA);
       const int rest = k0 + nb - (c0 + w);
       if (rest > 0) {
-        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, s, w, rest,
+        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, st, w, rest,
                            A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);
         const int mr = n - c0 - w;
         if (mr > 0)
-          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, s, mr,
+          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, st, mr,
                              rest, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA,
                              A + (long)(c0 + w) * lda + c0 + w, lda, sA);
       }
     }
+  };
+  // Lookahead: block k+1's strips run on a side stream once the trailing
+  // update has reached block k+1's columns, while the rest of that update
+  // runs on `s`. The interchanges of a block are applied to the columns
+  // outside it afterwards, on `s` (the strips touch only their own block).
+  const bool la = use_lookahead() && n > 2 * LU_NB;
+  LaCtx *cx = la ? la_for(s) : nullptr;
+  hipStream_t sp = la ? cx->side : s;
+  if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
+  if (la) {
+    HIPCHK(hipEventRecord(cx->ready, s));
+    HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
+  }
+  block_strips(0, min(LU_NB, n), sp);
+  if (la) HIPCHK(hipEventRecord(cx->panel, sp));
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int nb = min(LU_NB, n - k0);
+    if (la) HIPCHK(hipStreamWaitEvent(s, cx->panel, 0));
+    if (n > nb)
+      hipLaunchKernelGGL(k_ts_apply_defer<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, k0,
+                         (nb + PW - 1) / PW, A, lda, sA, fin0, finb, sW, k0, k0 + nb);
     const int nr = n - k0 - nb;
     if (nr > 0) {
       // U12 = L11^{-1} A12, PW rows at a time
@@ -845,9 +929,20 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
                              nr, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA,
                              A + (long)(c0 + w) * lda + k0 + nb, lda, sA);
       }
-      // A22 -= L21 U12
-      trailing_gemm<T>(nr, nr, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA, A + (long)k0 * lda + k0 + nb, lda,
+      // A22 -= L21 U12: the next block's columns first, then its strips, then the rest
+      const int nb1 = min(LU_NB, nr);
+      trailing_gemm<T>(nr, nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA, A + (long)k0 * lda + k0 + nb, lda,
                        sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s);
+      if (la) {
+        HIPCHK(hipEventRecord(cx->ready, s));
+        HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
+      }
+      block_strips(k0 + nb, nb1, sp);
+      if (la) HIPCHK(hipEventRecord(cx->panel, sp));
+      if (nr > nb1)
+        trailing_gemm<T>(nr, nr - nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA,
+                         A + (long)k0 * lda + k0 + nb + nb1, lda, sA, A + (long)(k0 + nb) * lda + k0 + nb + nb1, lda,
+                         sA, s);
     }
   }
   HIPCHK(hipGetLastError());
