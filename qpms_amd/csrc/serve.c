@@ -15,6 +15,7 @@
  * factorisation; the padded system's solution restricted to the first L
  * entries is the unpadded one.
  */
+#define _GNU_SOURCE   /* qsort_r */
 #include "qpms_internal.h"
 #include "../../include/qpms_hip_serve.h"
 #include <hip/hip_runtime_api.h>
@@ -815,10 +816,10 @@ static void retire(qpms_hip_server_t *s, qh_unit *u) {
  * are not held behind long contours while long ones still start
  * eventually. Ties keep arrival order. */
 #define SRV_AGING 0.1
-static double g_key_now;
-static int key_cmp(const void *a, const void *b) {
+static int key_cmp(const void *a, const void *b, void *now_) {
+	const double now = *(const double *)now_;
 	const qh_job *x = *(qh_job *const *)a, *y = *(qh_job *const *)b;
-	const double kx = x->est - SRV_AGING * (g_key_now - x->t_submit), ky = y->est - SRV_AGING * (g_key_now - y->t_submit);
+	const double kx = x->est - SRV_AGING * (now - x->t_submit), ky = y->est - SRV_AGING * (now - y->t_submit);
 	if (kx != ky) return kx < ky ? -1 : 1;
 	return x->id < y->id ? -1 : (x->id > y->id);
 }
@@ -839,8 +840,8 @@ static int try_launch(qpms_hip_server_t *s) {
 	}
 	size_t nq = 0;
 	for (qh_job *c = s->qhead; c; c = c->next) s->order[nq++] = c;
-	g_key_now = qpms_hip_server_clock(s);
-	qsort(s->order, nq, sizeof(*s->order), key_cmp);
+	double now = qpms_hip_server_clock(s);
+	qsort_r(s->order, nq, sizeof(*s->order), key_cmp, &now);
 	/* first job in that order that fits (backfilling past jobs that do not) */
 	for (size_t oi = 0; oi < nq; ++oi) {
 		qh_job *j = s->order[oi];
