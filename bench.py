@@ -119,6 +119,9 @@ def measure_w_build(L, ev, stream, nfreq=8):
             "traffic": W_BUILD_TRAFFIC, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": nbytes}
 
 
+LU_TRAFFIC_BYTES = 2.0774e11
+
+
 def measure_lu(L, ev, stream, n_part=546, reps=3):
     """FP32 LU of a mode-problem matrix of order 30 * n_part = 16380 (the
     serving loop's dominant work), HIP events on the library stream."""
@@ -157,7 +160,11 @@ def measure_lu(L, ev, stream, n_part=546, reps=3):
     flops = 8.0 / 3.0 * n ** 3
     ach = flops / (avg * 1e-3)
     return {"bound": "mfma", "achieved": ach / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": ach / FP32_MFMA_PEAK, "traffic": None,
+            "frac": ach / FP32_MFMA_PEAK,
+            # HBM bytes per factorisation from two rocprofv3 --pmc passes over this
+            # probe (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, copy-kernel calibration;
+            # profiles/lu_fp32_order16380.md)
+            "traffic": LU_TRAFFIC_BYTES,
             "kernel": "FP32 LU, order 16380 (tournament panels + trailing cgemm)",
             "avg_launch_ms": avg, "algorithmic_flops_per_launch": flops}
 
