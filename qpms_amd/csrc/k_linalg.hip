@@ -356,79 +356,64 @@ struct PanelWS {
 
 template <class T>
 struct TsShared {
-  T wv[2][TS_WAVES];
-  int wk[2][TS_WAVES], wr[2][TS_WAVES];
+  unsigned long long wp[2][TS_WAVES];   // each wave's packed winner
   int sel[PW_];    // block row of pivot j
   int skey[PW_];   // key (strip row) of pivot j
-  T sval[PW_];     // |U_jj|_1 at selection
+  T sval[PW_];     // 1 if the pivot j is not zero, else 0
 };
 
-// Arg-max over a wave of (a, key, row) with ties to the smaller key, by DPP
-// moves only: xor 1, xor 2, rotate 4, rotate 8 inside each 16-lane row, then
-// the row broadcasts; every lane returns lane 63's result.
+// Arg-max over a wave, by DPP moves only (xor 1, xor 2, rotate 4, rotate 8
+// inside each 16-lane row, then the row broadcasts; lane 63 ends with the
+// wave's winner) of one packed 64-bit key per lane:
+//   bits 63..32  the order bits of |x_j|_1 plus one (0: no live row) -- all 32
+//                bits of an FP32 value, the upper 32 of an FP64 one (order
+//                kept up to 2^-20 relative; nearer ties go to the key, as do
+//                exact ties in izamax's first-maximum rule);
+//   bits 31..10  0x3FFFFF - key (the smaller strip row wins a tie);
+//   bits  9..0   the row within the workgroup (< 512).
+// NaN magnitudes rank as zero. Non-negative floats order as their bits.
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
-template <int CTRL>
-__device__ __forceinline__ float dpp_t(float v) { return __int_as_float(dpp_i<CTRL>(__float_as_int(v))); }
-template <int CTRL>
-__device__ __forceinline__ double dpp_t(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float rdlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
-__device__ __forceinline__ double rdlane(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+__device__ __forceinline__ unsigned int ts_order_bits(float a) { return __float_as_uint(a); }
+__device__ __forceinline__ unsigned int ts_order_bits(double a) {
+  return (unsigned int)((unsigned long long)__double_as_longlong(a) >> 32);
 }
 template <class T>
-__device__ __forceinline__ void amax_pick(T &a, int &k, int &r, T a2, int k2, int r2) {
-  const bool b = (a2 > a) | ((a2 == a) & (k2 < k));
-  a = b ? a2 : a;
-  k = b ? k2 : k;
-  r = b ? r2 : r;
+__device__ __forceinline__ unsigned long long ts_pack(bool live, T a, int key, int row) {
+  if (!(a == a)) a = 0;
+  const unsigned int hi = live ? ts_order_bits(a) + 1u : 0u;
+  return ((unsigned long long)hi << 32) | ((unsigned long long)(0x3FFFFFu - (unsigned)key) << 10) | (unsigned)row;
 }
-template <class T, int CTRL>
-__device__ __forceinline__ void amax_dpp(T &a, int &k, int &r) {
-  amax_pick<T>(a, k, r, dpp_t<CTRL>(a), dpp_i<CTRL>(k), dpp_i<CTRL>(r));
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const unsigned lo = (unsigned)dpp_i<CTRL>((int)(unsigned)v), hi = (unsigned)dpp_i<CTRL>((int)(unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
 }
-// Row broadcasts (lane 15 of each row into the next row, lane 31 into the
-// upper half) for the rows the mask enables; disabled lanes keep `old`.
+// row broadcasts for the rows the mask enables; the others receive 0
 template <int CTRL, int RM>
-__device__ __forceinline__ int dppm_i(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xF, false); }
-template <int CTRL, int RM>
-__device__ __forceinline__ float dppm_t(float old, float v) {
-  return __int_as_float(dppm_i<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+__device__ __forceinline__ unsigned long long dppm_u64(unsigned long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, RM, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, RM, 0xF, false);
+  return ((unsigned long long)hi << 32) | lo;
 }
-template <int CTRL, int RM>
-__device__ __forceinline__ double dppm_t(double old, double v) {
-  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
-  const int lo = dppm_i<CTRL, RM>((int)o, (int)b), hi = dppm_i<CTRL, RM>((int)(o >> 32), (int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <class T, int CTRL, int RM>
-__device__ __forceinline__ void amax_bcast(T &a, int &k, int &r) {
-  amax_pick<T>(a, k, r, dppm_t<CTRL, RM>(T(-2), a), dppm_i<CTRL, RM>(0x7fffffff, k), dppm_i<CTRL, RM>(0, r));
-}
-template <class T>
-__device__ __forceinline__ void wave_amax(T &a, int &k, int &r) {
-  amax_dpp<T, 0xB1>(a, k, r);         // quad_perm [1,0,3,2]
-  amax_dpp<T, 0x4E>(a, k, r);         // quad_perm [2,3,0,1]
-  amax_dpp<T, 0x124>(a, k, r);        // row_ror 4
-  amax_dpp<T, 0x128>(a, k, r);        // row_ror 8: every lane of a row holds the row's winner
-  amax_bcast<T, 0x142, 0xA>(a, k, r); // row_bcast 15: rows 1, 3 take rows 0, 2
-  amax_bcast<T, 0x143, 0xC>(a, k, r); // row_bcast 31: rows 2, 3 take row 1; lane 63 holds the wave's
-  a = rdlane(a, 63);
-  k = __builtin_amdgcn_readlane(k, 63);
-  r = __builtin_amdgcn_readlane(r, 63);
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return b > a ? b : a; }
+__device__ __forceinline__ unsigned long long wave_amax(unsigned long long v) {
+  v = umax64(v, dpp_u64<0xB1>(v));          // quad_perm [1,0,3,2]
+  v = umax64(v, dpp_u64<0x4E>(v));          // quad_perm [2,3,0,1]
+  v = umax64(v, dpp_u64<0x124>(v));         // row_ror 4
+  v = umax64(v, dpp_u64<0x128>(v));         // row_ror 8: every lane of a row holds the row's winner
+  v = umax64(v, dppm_u64<0x142, 0xA>(v));   // row_bcast 15: rows 1, 3 take rows 0, 2
+  v = umax64(v, dppm_u64<0x143, 0xC>(v));   // row_bcast 31: rows 2, 3 take row 1
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 // Partial-pivoting elimination over the R rows of a workgroup staged in LDS
 // (X[row][XS]), used only to choose pivot rows. Row i belongs to thread i
 // (keys distinct, < 0 for an empty row). Per column: the wave arg-max of
-// |x_j|_1 (ties: smaller key, as izamax's first maximum), one LDS combine of
-// the waves (one barrier; the combine slots alternate between columns),
+// |x_j|_1 on packed keys, one LDS combine of the waves (one barrier; the
+// combine slots alternate between columns),
 // then every live row is reduced against the pivot row with the pivot's
 // reciprocal. A row's owner is the only writer of it, and the pivot row is
 // not written once chosen, so the next column's barrier orders the updates.
@@ -477,25 +462,21 @@ __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsS
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
     cT xj = cT{0, 0};
-    T a = -1;
-    if (live) {
-      xj = X[t * XS + j];
-      a = fabs(xj.x) + fabs(xj.y);
-      if (!(a == a)) a = 0;   // NaN: chosen only when nothing else is left
-    }
-    int k = live ? key : 0x7fffffff, r = t;
-    wave_amax<T>(a, k, r);
-    if (lane == 0) { sh.wv[par][wave] = a; sh.wk[par][wave] = k; sh.wr[par][wave] = r; }
+    if (live) xj = X[t * XS + j];
+    const unsigned long long v = wave_amax(ts_pack<T>(live, fabs(xj.x) + fabs(xj.y), live ? key : 0, t));
+    if (lane == 0) sh.wp[par][wave] = v;
     __syncthreads();
-    T pa = sh.wv[par][0];
-    int pk = sh.wk[par][0], pr = sh.wr[par][0];
+    unsigned long long m = sh.wp[par][0];
 #pragma unroll
-    for (int q = 1; q < TS_WAVES; ++q) amax_pick<T>(pa, pk, pr, sh.wv[par][q], sh.wk[par][q], sh.wr[par][q]);
-    if (pa < 0) break;   // no row left (uniform)
+    for (int q = 1; q < TS_WAVES; ++q) m = umax64(m, sh.wp[par][q]);
+    const unsigned int mhi = (unsigned int)(m >> 32);
+    if (mhi == 0) break;   // no row left (uniform)
+    const int pr = (int)(m & 0x3FFu), pk = (int)(0x3FFFFFu - (unsigned)((m >> 10) & 0x3FFFFFu));
+    const bool nonzero = mhi > 1u;   // the pivot's |x_j|_1 is not zero
     if (t == pr) live = false;
-    if (t == 0) { sh.sel[j] = pr; sh.skey[j] = pk; sh.sval[j] = pa; }
+    if (t == 0) { sh.sel[j] = pr; sh.skey[j] = pk; sh.sval[j] = nonzero ? T(1) : T(0); }
     ns = j + 1;
-    if (live && pa > 0) {
+    if (live && nonzero) {
       const cT *P = X + pr * XS;
       cT *Y = X + t * XS;
       const cT l = cmul<T>(xj, cdiv<T>(cT{1, 0}, P[j]));
@@ -838,6 +819,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
   if (n <= 0 || batch <= 0) return;
+  if (n >= (1 << 22)) qpms_hip_die(__FILE__, __LINE__, __func__, "order %d above the pivot search's 2^22 rows", n);
   const int Gmax = PanelWS<T>::sets(n);
   const long sW = (long)PanelWS<T>::bytes(n);
   const long setb = (long)PanelWS<T>::set_bytes();
