@@ -930,6 +930,90 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   HIPCHK(hipGetLastError());
 }
 
+// Triangular solves with few right-hand sides (nrhs <= TRV_R), in blocks of
+// TRV_B rows, on the permuted right-hand side W (n x nrhs, row-major):
+//   k_trsv_diag: one workgroup per matrix solves the w x w diagonal block
+//                (unit lower, forward; or upper, backward), thread i owning
+//                row r0 + i; per step the solved row is published through LDS
+//                (two slots alternate, one barrier per step) and every other
+//                row of the block subtracts its column entry times it;
+//   k_trsv_update: the rows outside the block (below it forward, above it
+//                backward) subtract the block's columns times its solution,
+//                one thread per row walking its row contiguously.
+constexpr int TRV_B = 256, TRV_R = 4;
+template <class T, bool FWD>
+__global__ __launch_bounds__(TRV_B) void k_trsv_diag(int r0, int w, int nrhs, const typename C2<T>::t *LU, long lda,
+                                                     long sA, typename C2<T>::t *W, long sW) {
+  typedef typename C2<T>::t cT;
+  __shared__ cT sx[2][TRV_R];
+  LU += (long)blockIdx.z * sA;
+  W += (long)blockIdx.z * sW;
+  const int i = threadIdx.x;
+  const bool mine = i < w;
+  cT y[TRV_R];
+#pragma unroll
+  for (int c = 0; c < TRV_R; ++c) y[c] = (mine && c < nrhs) ? W[(long)(r0 + i) * nrhs + c] : cT{0, 0};
+  const cT *row = LU + (long)(r0 + (mine ? i : 0)) * lda + r0;
+  for (int s = 0; s < w; ++s) {
+    const int j = FWD ? s : w - 1 - s, par = s & 1;
+    if (i == j) {
+      cT d = cT{1, 0};
+      if (!FWD) d = cdiv<T>(cT{1, 0}, row[j]);
+#pragma unroll
+      for (int c = 0; c < TRV_R; ++c) {
+        if (!FWD) y[c] = cmul<T>(y[c], d);
+        sx[par][c] = y[c];
+      }
+    }
+    __syncthreads();
+    if (mine && (FWD ? i > j : i < j)) {
+      const cT l = row[j];
+#pragma unroll
+      for (int c = 0; c < TRV_R; ++c) {
+        const cT x = sx[par][c];
+        y[c] = cT{y[c].x - (l.x * x.x - l.y * x.y), y[c].y - (l.x * x.y + l.y * x.x)};
+      }
+    }
+  }
+  if (mine)
+#pragma unroll
+    for (int c = 0; c < TRV_R; ++c)
+      if (c < nrhs) W[(long)(r0 + i) * nrhs + c] = y[c];
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_trsv_update(int i0, int nrows, int r0, int w, int nrhs,
+                                                    const typename C2<T>::t *LU, long lda, long sA,
+                                                    typename C2<T>::t *W, long sW) {
+  typedef typename C2<T>::t cT;
+  __shared__ cT sx[TRV_B * TRV_R];
+  LU += (long)blockIdx.z * sA;
+  W += (long)blockIdx.z * sW;
+  for (int e = threadIdx.x; e < w * nrhs; e += 64) sx[(e / nrhs) * TRV_R + e % nrhs] = W[(long)r0 * nrhs + e];
+  __syncthreads();
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= nrows) return;
+  const int i = i0 + r;
+  const cT *row = LU + (long)i * lda + r0;
+  cT acc[TRV_R];
+#pragma unroll
+  for (int c = 0; c < TRV_R; ++c) acc[c] = cT{0, 0};
+  for (int j = 0; j < w; ++j) {
+    const cT l = row[j];
+#pragma unroll
+    for (int c = 0; c < TRV_R; ++c) {
+      const cT x = sx[j * TRV_R + c];
+      acc[c] = cT{acc[c].x + (l.x * x.x - l.y * x.y), acc[c].y + (l.x * x.y + l.y * x.x)};
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TRV_R; ++c)
+    if (c < nrhs) {
+      cT &y = W[(long)i * nrhs + c];
+      y = cT{y.x - acc[c].x, y.y - acc[c].y};
+    }
+}
+
 // B (n x nrhs, ldb) <- A^{-1} B given LU and the row permutation.
 template <class T>
 static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *LU, long lda, long sA,
@@ -941,6 +1025,27 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
   // work <- P B, then forward / backward substitution in place on work, copy back.
   hipLaunchKernelGGL(k_gather_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs, perm, sP,
                      (const cT *)B, ldb, sB, work, (long)nrhs, sW);
+  if (nrhs <= TRV_R) {
+    const int nblk = (n + TRV_B - 1) / TRV_B;
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int r0 = bi * TRV_B, w = min(TRV_B, n - r0), rem = n - r0 - w;
+      hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1, 1, batch), dim3(TRV_B), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      if (rem > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((rem + 63) / 64, 1, batch), dim3(64), 0, s, r0 + w, rem, r0, w, nrhs,
+                           LU, lda, sA, work, sW);
+    }
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+      const int r0 = bi * TRV_B, w = min(TRV_B, n - r0);
+      hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1, 1, batch), dim3(TRV_B), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      if (r0 > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((r0 + 63) / 64, 1, batch), dim3(64), 0, s, 0, r0, r0, w, nrhs, LU, lda,
+                           sA, work, sW);
+    }
+    hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs,
+                       (const cT *)work, (long)nrhs, sW, B, ldb, sB);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   for (int r0 = 0; r0 < n; r0 += NBW) {
     const int w = (n - r0 < NBW) ? n - r0 : NBW;
     hipLaunchKernelGGL((k_trsm<T, true>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
