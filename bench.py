@@ -463,6 +463,8 @@ def main():
     lat = np.array([t - reqs[i].arrival for i, (r, t) in ok.items()]) if ok else np.array([np.nan])
     value = len(ok) / win_total
     kinds = {k: sum(1 for i in ok if reqs[i].kind == k) for k in (0, 1, 2)}
+    bsz = np.array([r.batch for r, _ in ok.values() if r.kind == sv.KIND_SCATTER] or [0])
+    log(f"plane-wave solves by batch size: " + ", ".join(f"{b}: {int(np.sum(bsz == b))}" for b in np.unique(bsz)))
     log(f"served {len(ok)} in window ({kinds}), {len(late)} after, {cancelled} cancelled; "
         f"p50 {np.percentile(lat, 50):.3f} s p99 {np.percentile(lat, 99):.3f} s")
 
@@ -486,7 +488,9 @@ def main():
             if time.perf_counter() - t_start > args.plain_budget:
                 break
         n_p = len(times)
-        plain = {"solves_per_s": n_p / sum(times), "requests": n_p,
+        plain_s_by_kind = {str(k): float(sum(t for t, kk in zip(times, kinds_p) if kk == k)) for k in (0, 1, 2)}
+        plain = {"solves_per_s": n_p / sum(times), "requests": n_p, "of_served": len(ok),
+                 "busy_s_by_kind": plain_s_by_kind,
                  "p50_s": float(np.percentile(times, 50)), "p99_s": float(np.percentile(times, 99)),
                  "max_rel_diff_served_vs_fp64": float(np.max(errs)),
                  "max_rel_diff_by_kind": {str(k): float(max([e for e, kk in zip(errs, kinds_p) if kk == k] or [0]))

@@ -77,6 +77,8 @@ typedef struct qpms_hip_response_t {
 	double sv[QPMS_HIP_SERVE_MAX_EIG];  /* BEYN: singular values of A0 (beyn_l of them) */
 	double t_submit, t_start, t_done;   /* seconds on the server's clock */
 	char message[96];
+	int batch;                  /* requests factorised together in this request's unit */
+	int padded_dim;             /* order the request was padded to (batched solves) */
 } qpms_hip_response_t;
 
 typedef struct qpms_hip_server_opts_t {

@@ -815,7 +815,7 @@ static size_t getrf_ws_bytes(int n, int batch) {
 // (single-stream callers only).
 template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
-                          int *perm, int *dinfo, hipStream_t s, char *ws = nullptr) {
+                          int *perm, int *dinfo, hipStream_t s, char *ws = nullptr, bool lookahead = true) {
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
   if (n <= 0 || batch <= 0) return;
@@ -882,7 +882,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // update has reached block k+1's columns, while the rest of that update
   // runs on `s`. The interchanges of a block are applied to the columns
   // outside it afterwards, on `s` (the strips touch only their own block).
-  const bool la = use_lookahead() && n > 2 * LU_NB;
+  const bool la = lookahead && use_lookahead() && n > 2 * LU_NB;
   LaCtx *cx = la ? la_for(s) : nullptr;
   hipStream_t sp = la ? cx->side : s;
   if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
@@ -1118,14 +1118,18 @@ extern "C" int qh_getrf_batched_f64(int n, int batch, double *A, long sA, int *i
 extern "C" size_t qh_getrf_ws_bytes(int prec, int n, int batch) {
   return prec == 32 ? getrf_ws_bytes<float>(n, batch) : getrf_ws_bytes<double>(n, batch);
 }
+// Caller-workspace entry points: their callers (the serving loop's lanes)
+// already overlap work across several streams, so no lookahead side stream.
 extern "C" int qh_getrf_batched_ws_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info, void *ws,
                                        void *stream) {
-  getrf_batched<double>(n, batch, (v2d *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws);
+  getrf_batched<double>(n, batch, (v2d *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
+                        false);
   return 0;
 }
 extern "C" int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info, void *ws,
                                        void *stream) {
-  getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws);
+  getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
+                       false);
   return 0;
 }
 extern "C" int qh_getrs_batched_f64(int n, int nrhs, int batch, const double *LU, long sA, const int *perm,

@@ -792,6 +792,8 @@ static void retire(qpms_hip_server_t *s, qh_unit *u) {
 		out[j].dim = u->jobs[j]->L;
 		out[j].t_submit = u->jobs[j]->t_submit;
 		out[j].t_start = u->t_start;
+		out[j].batch = u->njobs;
+		out[j].padded_dim = (int)u->npad;
 	}
 	if (u->kind == U_SCATTER) finish_scatter(s, u, out);
 	else if (u->kind == U_PERIODIC) finish_periodic(u, out);

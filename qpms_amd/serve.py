@@ -40,7 +40,8 @@ class Response(ctypes.Structure):
                 ("refine_steps", ctypes.c_int), ("logabsdet", ctypes.c_double), ("argdet", ctypes.c_double),
                 ("neig", ctypes.c_int), ("eigval", c_cplx * MAX_EIG), ("sv", ctypes.c_double * MAX_EIG),
                 ("t_submit", ctypes.c_double),
-                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("message", ctypes.c_char * 96)]
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("message", ctypes.c_char * 96),
+                ("batch", ctypes.c_int), ("padded_dim", ctypes.c_int)]
 
     def eigenvalues(self):
         return np.array([complex(self.eigval[i].re, self.eigval[i].im) for i in range(self.neig)])

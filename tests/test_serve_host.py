@@ -20,7 +20,7 @@ def c_layout():
                                "rank_tol", "beyn_centre_im"],
         "qpms_hip_response_t": ["id", "status", "kind", "dim", "sigma_ext", "rel_residual", "refine_steps",
                                 "logabsdet", "argdet", "neig", "eigval", "sv", "t_submit", "t_start", "t_done",
-                                "message"],
+                                "message", "batch", "padded_dim"],
         "qpms_hip_server_opts_t": ["arena_bytes", "nstreams", "mixed", "batch_max_dim", "batch_max", "refine_steps"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "qpms_hip_serve.h"', "int main(void) {"]
