@@ -177,6 +177,8 @@ def replay(stream_reqs, window, mixed=True, arena_gib=160, drain_cancel=True):
     id2i = {}
     got = {}
     nxt = 0
+    if drain_cancel:   # start nothing that is not estimated to finish inside the window
+        srv.set_deadline(srv.clock() + window)
     t0 = time.perf_counter()
     last_log = t0
     while True:
@@ -194,11 +196,13 @@ def replay(stream_reqs, window, mixed=True, arena_gib=160, drain_cancel=True):
             last_log = time.perf_counter()
         time.sleep(0.0005)
     if drain_cancel:
-        srv.cancel()
-    while srv.pending():
-        srv.progress(wait=True)
+        srv.cancel()   # queued requests are answered at once (status -2)
+    while True:
         for r in srv.poll():
             got[id2i[r.id]] = (r, time.perf_counter() - t0)
+        if not srv.pending():
+            break
+        srv.progress(wait=True)
     wall = time.perf_counter() - t0
     srv.close()
     return got, nxt, wall
@@ -390,6 +394,32 @@ def per_kind(n, plain=True):
                         for k, a in tot.items()}}
 
 
+def burst(n):
+    """The requests of per_kind(n) submitted at once: the loop's wall time to
+    drain them, against their served-alone times (per_kind's served_s)."""
+    from qpms_amd import serve as sv
+    reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
+            if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
+    srv = sv.Server(arena_gib=160, nstreams=4, mixed=True)
+    for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
+        srv.submit(r)
+    while srv.pending():
+        srv.progress(wait=True)
+        srv.poll()
+    t0 = time.perf_counter()
+    for r in reqs:
+        srv.submit(r)
+    got = []
+    while srv.pending():
+        srv.progress(wait=True)
+        got += srv.poll()
+    wall = time.perf_counter() - t0
+    srv.close()
+    bsz = [r.batch for r in got if r.kind == sv.KIND_SCATTER]
+    return {"metric": "burst drain wall time", "requests": n, "wall_s": wall, "answered": len(got),
+            "batch_sizes": sorted(bsz)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["serving", "ewald", "beyn"], default="serving")
@@ -397,7 +427,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--window", type=float, default=45.0, help="seconds of stream replayed per step")
-    ap.add_argument("--rate", type=float, default=10.0,
+    ap.add_argument("--rate", type=float, default=25.0,
                     help="request arrival rate (1/s); above the loop's capacity, so the window measures capacity")
     ap.add_argument("--plain-budget", type=float, default=150.0, help="seconds for the plain FP64 comparison")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -406,6 +436,7 @@ def main():
     ap.add_argument("--no-wbuild", action="store_true")
     ap.add_argument("--lu-probe", action="store_true", help="only the FP32 LU measurement (for rocprofv3)")
     ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
+    ap.add_argument("--burst", type=int, default=0, help="submit the per-kind requests at once and drain them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); requests would be "
@@ -422,6 +453,9 @@ def main():
 
     if args.lu_probe:
         print(json.dumps(measure_lu(L, ev, stream, reps=2)), flush=True)
+        return
+    if args.burst:
+        print(json.dumps(burst(args.burst)), flush=True)
         return
     if args.per_kind:
         print(json.dumps(per_kind(args.per_kind, plain=not args.no_plain)), flush=True)

@@ -112,6 +112,10 @@ size_t qpms_hip_server_pending(const qpms_hip_server_t *srv);
  * in flight stop after their current chunk of nodes (status -2); other work
  * in flight completes normally. Keep calling progress until pending is 0. */
 void qpms_hip_server_cancel(qpms_hip_server_t *srv);
+/* Start no request after time t (server clock, seconds) unless its estimated
+ * device time fits before t (0: no deadline). Requests already running are
+ * not affected; those that do not fit stay queued. */
+void qpms_hip_server_set_deadline(qpms_hip_server_t *srv, double t);
 
 /* The same request answered through the public scattering-system API one
  * call after another, with host buffers and FP64 LU (the plain drop-in path

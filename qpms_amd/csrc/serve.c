@@ -193,6 +193,7 @@ struct qpms_hip_server_t {
 	qpms_hip_response_t *ready;
 	int nready, capready;
 	qh_job **order;        /* launch-order scratch */
+	double deadline;       /* 0, or: start only what is estimated to finish by then */
 	size_t order_cap;
 	unsigned long long next_id;
 	struct timespec t0;
@@ -847,6 +848,8 @@ static int try_launch(qpms_hip_server_t *s) {
 	/* first job in that order that fits (backfilling past jobs that do not) */
 	for (size_t oi = 0; oi < nq; ++oi) {
 		qh_job *j = s->order[oi];
+		/* 1.5: margin on the estimate, which ignores contention with other lanes */
+		if (s->deadline > 0 && now + 1.5 * j->est > s->deadline) continue;
 		int chunk = 1;
 		const int small = j->req.kind == QPMS_HIP_REQ_SCATTER && j->L <= s->o.batch_max_dim;
 		qh_unit *u = unit_new(s);
@@ -932,6 +935,8 @@ int qpms_hip_server_poll(qpms_hip_server_t *s, qpms_hip_response_t *out, int max
 	s->nready -= n;
 	return n;
 }
+
+void qpms_hip_server_set_deadline(qpms_hip_server_t *s, double t) { s->deadline = t; }
 
 void qpms_hip_server_cancel(qpms_hip_server_t *s) {
 	const double now = qpms_hip_server_clock(s);

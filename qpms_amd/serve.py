@@ -71,6 +71,7 @@ def _bind():
     L.qpms_hip_server_pending.restype = ctypes.c_size_t
     L.qpms_hip_server_pending.argtypes = [vp]
     L.qpms_hip_server_cancel.argtypes = [vp]
+    L.qpms_hip_server_set_deadline.argtypes = [vp, ctypes.c_double]
     L.qpms_hip_request_plain.restype = ctypes.c_int
     L.qpms_hip_request_plain.argtypes = [ctypes.POINTER(Request), ctypes.POINTER(Response)]
     L._serve_bound = True
@@ -106,6 +107,10 @@ class Server:
 
     def cancel(self):
         self.L.qpms_hip_server_cancel(self.h)
+
+    def set_deadline(self, t):
+        """Start nothing estimated to finish after t (server clock, s); 0 clears."""
+        self.L.qpms_hip_server_set_deadline(self.h, ctypes.c_double(t))
 
     def close(self):
         if self.h:

@@ -161,3 +161,31 @@ def test_admission_by_footprint(gpu):
     assert sorted(got) == sorted(ids)
     for r, i in zip(reqs, ids):
         close(got[i], sv.request_plain(r), r.kind, 1e-9)
+
+
+def test_deadline_keeps_long_requests_queued(gpu):
+    """With a deadline, a request whose estimate does not fit stays queued
+    (cancelled when the caller gives up) while short ones run."""
+    from qpms_amd import serve as sv
+    rng = np.random.default_rng(5)
+    srv = sv.Server(arena_gib=48, nstreams=4, mixed=True)
+    srv.set_deadline(srv.clock() + 0.5)
+    big, _ = grid(600, 400e-9, rng)   # order 600 * 48 = 28800 at lMax 4: seconds of work
+    small, _ = grid(20, 400e-9, rng)
+    rb = srv.submit(sv.Req(sv.KIND_SCATTER, big, 4, 50e-9, -12 + 1j, 2.31, 2 * EV))
+    rs = srv.submit(sv.Req(sv.KIND_SCATTER, small, 2, 50e-9, -12 + 1j, 2.31, 2 * EV))
+    got = {}
+    while rs not in got:
+        srv.progress(wait=True)
+        for r in srv.poll():
+            got[r.id] = r
+    assert got[rs].status == 0 and rb not in got and srv.pending() == 1
+    srv.cancel()   # queued requests are answered at once (status -2)
+    while True:
+        for r in srv.poll():
+            got[r.id] = r
+        if not srv.pending():
+            break
+        srv.progress(wait=True)
+    srv.close()
+    assert got[rb].status == -2
