@@ -945,6 +945,7 @@ template <class T, bool FWD>
 __global__ __launch_bounds__(TRV_B) void k_trsv_diag(int r0, int w, int nrhs, const typename C2<T>::t *LU, long lda,
                                                      long sA, typename C2<T>::t *W, long sW) {
   typedef typename C2<T>::t cT;
+  constexpr int PF = 4;   // row entries loaded this many steps ahead (off the per-step chain)
   __shared__ cT sx[2][TRV_R];
   LU += (long)blockIdx.z * sA;
   W += (long)blockIdx.z * sW;
@@ -954,24 +955,34 @@ __global__ __launch_bounds__(TRV_B) void k_trsv_diag(int r0, int w, int nrhs, co
 #pragma unroll
   for (int c = 0; c < TRV_R; ++c) y[c] = (mine && c < nrhs) ? W[(long)(r0 + i) * nrhs + c] : cT{0, 0};
   const cT *row = LU + (long)(r0 + (mine ? i : 0)) * lda + r0;
-  for (int s = 0; s < w; ++s) {
-    const int j = FWD ? s : w - 1 - s, par = s & 1;
-    if (i == j) {
-      cT d = cT{1, 0};
-      if (!FWD) d = cdiv<T>(cT{1, 0}, row[j]);
+  // backward: each row's own diagonal reciprocal, computed up front
+  const cT dinv = (!FWD && mine) ? cdiv<T>(cT{1, 0}, row[i]) : cT{1, 0};
+  auto col = [&](int s) { return FWD ? s : w - 1 - s; };
+  cT buf[PF];
 #pragma unroll
-      for (int c = 0; c < TRV_R; ++c) {
-        if (!FWD) y[c] = cmul<T>(y[c], d);
-        sx[par][c] = y[c];
+  for (int k = 0; k < PF; ++k) buf[k] = (mine && k < w) ? row[col(k)] : cT{0, 0};
+  for (int s0 = 0; s0 < w; s0 += PF) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int s = s0 + k;
+      if (s >= w) continue;   // uniform
+      const int j = col(s), par = s & 1;
+      const cT l = buf[k];
+      buf[k] = (mine && s + PF < w) ? row[col(s + PF)] : cT{0, 0};
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < TRV_R; ++c) {
+          if (!FWD) y[c] = cmul<T>(y[c], dinv);
+          sx[par][c] = y[c];
+        }
       }
-    }
-    __syncthreads();
-    if (mine && (FWD ? i > j : i < j)) {
-      const cT l = row[j];
+      __syncthreads();
+      if (mine && (FWD ? i > j : i < j)) {
 #pragma unroll
-      for (int c = 0; c < TRV_R; ++c) {
-        const cT x = sx[par][c];
-        y[c] = cT{y[c].x - (l.x * x.x - l.y * x.y), y[c].y - (l.x * x.y + l.y * x.x)};
+        for (int c = 0; c < TRV_R; ++c) {
+          const cT x = sx[par][c];
+          y[c] = cT{y[c].x - (l.x * x.x - l.y * x.y), y[c].y - (l.x * x.y + l.y * x.x)};
+        }
       }
     }
   }
