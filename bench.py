@@ -229,18 +229,19 @@ def cpu_baseline_scatter(reqs, budget_s, threads):
     from qpms_amd import serve as sv
     o = oracle_ffi.oracle()
     o.or_set_threads(threads)
-    done, t_used = 0, 0.0
-    for r in reqs:
+    done, t_used, sigma = 0, 0.0, {}
+    for i, r in enumerate(reqs):
         if r.kind != sv.KIND_SCATTER or r.dim > CPU_MAX_DIM:
             continue
         t0 = time.perf_counter()
-        oracle_ffi.finite_cluster(r.pos, r.c.lMax, r.c.radius, complex(r.c.eps_in.re, r.c.eps_in.im), r.c.eps_bg,
-                                  r.c.omega / EV, (0, 0, 1), np.array([1, 0, 0], complex))
+        res = oracle_ffi.finite_cluster(r.pos, r.c.lMax, r.c.radius, complex(r.c.eps_in.re, r.c.eps_in.im),
+                                        r.c.eps_bg, r.c.omega / EV, (0, 0, 1), np.array([1, 0, 0], complex))
         t_used += time.perf_counter() - t0
+        sigma[i] = float(res["sigma_ext"])
         done += 1
         if t_used > budget_s:
             break
-    return done, t_used
+    return done, t_used, sigma
 
 
 def bench_ewald(L, ev, stream):
@@ -535,13 +536,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        n_c, t_c = cpu_baseline_scatter(reqs, args.cpu_budget, threads)
+        n_c, t_c, sig_o = cpu_baseline_scatter(reqs, args.cpu_budget, threads)
+        # the served answers of those requests against the oracle's
+        cmp = [abs(ok[i][0].sigma_ext - so) / abs(so) for i, so in sig_o.items() if i in ok]
         cpu = {"value": n_c / t_c if t_c > 0 else None, "unit": "plane-wave solves/s", "cores": threads,
                "kind": "port",
+               "served_vs_oracle": {"requests": len(cmp), "max_rel_diff": float(max(cmp)) if cmp else None},
                "sample": f"oracle pipeline (W, I - T W, LU, extinction) on the first {n_c} plane-wave requests of "
                          f"order <= {CPU_MAX_DIM} in the stream ({t_c:.1f} s); the GPU figure covers every order "
                          f"and also determinants and contours"}
-        log(f"cpu baseline: {n_c} plane-wave solves in {t_c:.1f} s")
+        log(f"cpu baseline: {n_c} plane-wave solves in {t_c:.1f} s; served vs oracle on {len(cmp)} of them: "
+            f"max rel diff {max(cmp) if cmp else float('nan'):.2e}")
 
     line = {
         "metric": METRIC,
