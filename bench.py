@@ -173,7 +173,7 @@ def replay(stream_reqs, window, mixed=True, arena_gib=160, drain_cancel=True):
     """Submit requests at their arrival times for `window` seconds; returns
     (responses by index, requests submitted, wall seconds until drained)."""
     from qpms_amd import serve as sv
-    srv = sv.Server(arena_gib=arena_gib, nstreams=4, mixed=mixed)
+    srv = sv.Server(arena_gib=arena_gib, nstreams=8, mixed=mixed)
     id2i = {}
     got = {}
     nxt = 0
@@ -363,7 +363,7 @@ def per_kind(n, plain=True):
     from qpms_amd import serve as sv
     reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
             if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
-    srv = sv.Server(arena_gib=160, nstreams=4, mixed=True)
+    srv = sv.Server(arena_gib=160, nstreams=8, mixed=True)
     for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
         srv.submit(r)
     while srv.pending():
@@ -401,7 +401,7 @@ def burst(n):
     from qpms_amd import serve as sv
     reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
             if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
-    srv = sv.Server(arena_gib=160, nstreams=4, mixed=True)
+    srv = sv.Server(arena_gib=160, nstreams=8, mixed=True)
     for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
         srv.submit(r)
     while srv.pending():

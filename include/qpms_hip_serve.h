@@ -83,7 +83,7 @@ typedef struct qpms_hip_response_t {
 
 typedef struct qpms_hip_server_opts_t {
 	size_t arena_bytes;         /* device arena for resident requests (0: 64 GiB) */
-	int nstreams;               /* concurrent streams (0: 4) */
+	int nstreams;               /* concurrent streams, at most 8 (0: 8) */
 	int mixed;                  /* 1: FP32 LU + FP64 refinement for solves; 0: FP64 LU */
 	size_t batch_max_dim;       /* plane-wave solves with L <= this are batched (0: 8192) */
 	int batch_max;              /* most requests per batch (0: 32) */

@@ -268,7 +268,7 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 	QH_CALLOC(s, 1, sizeof(*s));
 	if (opts) s->o = *opts;
 	if (!s->o.arena_bytes) s->o.arena_bytes = (size_t)64 << 30;
-	if (s->o.nstreams <= 0) s->o.nstreams = 4;
+	if (s->o.nstreams <= 0) s->o.nstreams = 8;
 	if (s->o.nstreams > MAX_LANES) s->o.nstreams = MAX_LANES;
 	if (!s->o.batch_max_dim) s->o.batch_max_dim = 8192;
 	if (s->o.batch_max <= 0) s->o.batch_max = 32;

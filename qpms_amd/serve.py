@@ -79,7 +79,7 @@ def _bind():
 
 
 class Server:
-    def __init__(self, arena_gib=64, nstreams=4, mixed=True, batch_max_dim=8192, batch_max=32, refine_steps=3):
+    def __init__(self, arena_gib=64, nstreams=8, mixed=True, batch_max_dim=8192, batch_max=32, refine_steps=3):
         self.L = _bind()
         o = Opts(int(arena_gib * (1 << 30)), nstreams, int(mixed), batch_max_dim, batch_max, refine_steps)
         self.h = self.L.qpms_hip_server_create(ctypes.byref(o))
