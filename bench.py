@@ -119,6 +119,64 @@ def measure_w_build(L, ev, stream, nfreq=8):
             "traffic": W_BUILD_TRAFFIC, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": nbytes}
 
 
+def measure_peaks(L, ev, stream):
+    """Measured counterparts of the datasheet peaks: STREAM triad over three
+    2 GiB arrays (hand-written kernel), FP64 FMA chains (hand-written kernel),
+    and rocBLAS cgemm / zgemm at order 8192 (the library GEMM the LU's
+    trailing update calls)."""
+    import qpms_amd
+    L.qpms_hip_probe_triad.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.c_void_p]
+    L.qpms_hip_probe_fma64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+
+    def timed(fn, reps):
+        fn()
+        L.qpms_hip_sync()
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream)
+        for _ in range(reps):
+            fn()
+        ev.record(b, stream)
+        return ev.elapsed_ms(a, b) / reps
+
+    out = {}
+    n = 1 << 28
+    bufs = [qpms_amd.DeviceBuffer(8 * n) for _ in range(3)]
+    ms = timed(lambda: L.qpms_hip_probe_triad(n, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, 0.5, None), 5)
+    out["hbm_triad_GBps"] = 24.0 * n / (ms * 1e-3) / 1e9
+    for b in bufs:
+        b.free()
+    blocks, iters = 256 * 8, 1 << 15
+    o = qpms_amd.DeviceBuffer(blocks * 256 * 8)
+    ms = timed(lambda: L.qpms_hip_probe_fma64(blocks, iters, o.ptr, None), 3)
+    out["fp64_fma_TFps"] = 16.0 * iters * 256 * blocks / (ms * 1e-3) / 1e12
+    o.free()
+    try:
+        rb = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "librocblas.so"))
+        h = ctypes.c_void_p()
+        assert rb.rocblas_create_handle(ctypes.byref(h)) == 0
+        assert rb.rocblas_set_stream(h, ctypes.c_void_p(stream)) == 0
+        nn = 8192
+        for name, fn, esz in (("cgemm", rb.rocblas_cgemm, 8), ("zgemm", rb.rocblas_zgemm, 16)):
+            A, B, C = (qpms_amd.DeviceBuffer(nn * nn * esz) for _ in range(3))
+            one = (ctypes.c_double * 2)(1, 0) if esz == 16 else (ctypes.c_float * 2)(1, 0)
+            zero = (ctypes.c_double * 2)(0, 0) if esz == 16 else (ctypes.c_float * 2)(0, 0)
+
+            def g():
+                assert fn(h, 111, 111, nn, nn, nn, one, ctypes.c_void_p(A.ptr), nn, ctypes.c_void_p(B.ptr), nn, zero,
+                          ctypes.c_void_p(C.ptr), nn) == 0
+            ms = timed(g, 3)
+            out[name + "_TFps"] = 8.0 * nn ** 3 / (ms * 1e-3) / 1e12
+            for b in (A, B, C):
+                b.free()
+        rb.rocblas_destroy_handle(h)
+    except (OSError, AttributeError, AssertionError) as e:
+        log(f"rocBLAS GEMM probe skipped: {e}")
+    out["datasheet"] = {"hbm_GBps": HBM_PEAK / 1e9, "fp64_vector_TFps": 78.6, "fp32_matrix_TFps": 157.3,
+                        "fp64_matrix_TFps": 78.6}
+    return out
+
+
 LU_TRAFFIC_BYTES = 2.0774e11
 
 
@@ -311,6 +369,54 @@ def bench_ewald(L, ev, stream):
             "parity_max_rel_err_sample": err}
 
 
+def bench_field(L, ev, stream):
+    """Scattered-field map: the field of a 32 x 32 square array (pitch 400 nm,
+    lMax 3, seeded random outgoing-wave coefficients) on a 256 x 256 grid of
+    points 200 nm above it, FP64; one k_scattered_e launch per map."""
+    import qpms_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as of
+    lMax, ne = 3, 15
+    g = (np.arange(32) - 15.5) * 400e-9
+    pos = np.array([(x, y, 0.0) for x in g for y in g])
+    rng = np.random.default_rng(5)
+    cv = (rng.normal(size=(len(pos), 2 * ne)) + 1j * rng.normal(size=(len(pos), 2 * ne))) * 1e-2
+    u = np.linspace(-7e-6, 7e-6, 256)
+    pts = np.array([(x, y, 200e-9) for x in u for y in u])
+    k = 1.52 * 2.0 * EV / C0
+    dp, dc, dw = (qpms_amd.DeviceBuffer.from_array(a) for a in (pos, cv, pts))
+    do = qpms_amd.DeviceBuffer(len(pts) * 48)
+    L.qpms_hip_scattered_E_dev.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, qpms_amd.c_cplx,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+
+    def run():
+        L.qpms_hip_scattered_E_dev(lMax, len(pos), dp.ptr, qpms_amd.c_cplx.of(k), 3, dc.ptr, len(pts), dw.ptr,
+                                   do.ptr, 0, None)
+    run()
+    L.qpms_hip_sync()
+    a, b = ev.create(), ev.create()
+    ev.record(a, stream)
+    reps = 5
+    for _ in range(reps):
+        run()
+    ev.record(b, stream)
+    ms = ev.elapsed_ms(a, b) / reps
+    got = do.to_array(np.complex128, (len(pts), 3))
+    idx = rng.choice(len(pts), 48, replace=False)
+    t0 = time.perf_counter()
+    ref = of.scattered_E(lMax, pos, k, cv.ravel(), pts[idx])
+    t_cpu = time.perf_counter() - t0
+    err = float(np.max(np.abs(got[idx] - ref)) / np.max(np.abs(ref)))
+    for x in (dp, dc, dw, do):
+        x.free()
+    pairs = len(pts) * len(pos)
+    return {"metric": "scattered-field (point, particle) pairs/s (FP64)", "value": pairs / (ms * 1e-3),
+            "unit": "pairs/s", "points": len(pts), "particles": len(pos), "lMax": lMax, "ms_per_map": ms,
+            "max_rel_diff_vs_oracle": err, "oracle_sample_points": len(idx),
+            "cpu_oracle_pairs_per_s_1thread": len(idx) * len(pos) / t_cpu}
+
+
 def bench_beyn(L, window=None):
     """Beyn contour on a 24 x 24 cluster (pitch 400 nm, r 40 nm, eps -12+1i,
     n 1.52, lMax 2: m = 9216), ellipse 2.05 eV +- 0.05 eV (Re) x 0.005 eV
@@ -423,7 +529,7 @@ def burst(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["serving", "ewald", "beyn"], default="serving")
+    ap.add_argument("--workload", choices=["serving", "ewald", "beyn", "peaks", "field"], default="serving")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -464,10 +570,18 @@ def main():
     if args.workload == "ewald":
         print(json.dumps(bench_ewald(L, ev, stream)), flush=True)
         return
+    if args.workload == "peaks":
+        print(json.dumps(measure_peaks(L, ev, stream)), flush=True)
+        return
+    if args.workload == "field":
+        print(json.dumps(bench_field(L, ev, stream)), flush=True)
+        return
     if args.workload == "beyn":
         print(json.dumps(bench_beyn(L)), flush=True)
         return
 
+    peaks = measure_peaks(L, ev, stream)
+    log("measured peaks: " + ", ".join(f"{k} {v:.1f}" for k, v in peaks.items() if k != "datasheet"))
     wb = None if args.no_wbuild else measure_w_build(L, ev, stream)
     if wb:
         log(f"W build: {wb['achieved']:.0f} GB/s ({100 * wb['frac']:.1f} % of HBM peak)")
@@ -557,7 +671,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 LU + c128 refinement (solves), c128 (builds, Ewald, determinants, Beyn SVD)",
         "data": "synthetic (seeded request stream)",
-        "config": {"workload": "serving stream (configs[4]): Poisson arrivals, N log-uniform 16..1024, lMax 2-4, "
+        "config": {"workload": "serving stream: Poisson arrivals, N log-uniform 16..1024, lMax 2-4, "
                                "70/20/10 plane-wave / periodic-determinant / 32-node Beyn",
                    "rate_per_s": args.rate, "window_s": args.window, "requests_submitted": submitted,
                    "served_in_window": len(ok), "served_by_kind": kinds, "served_after_window": len(late),
@@ -567,6 +681,7 @@ def main():
         "vs_plain_fp64": (value / plain["solves_per_s"]) if plain else None,
         "roofline": roof,
         "w_build": wb,
+        "measured_peaks": peaks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

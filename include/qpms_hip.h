@@ -532,6 +532,37 @@ _Complex double *qpms_scatsysw_apply_Tmatrices_full(_Complex double *target_full
 		const qpms_scatsys_at_omega_t *ssw);
 
 /* ------------------------------------------------------------------------ */
+/* Scattered field at points — scatsystem.h:704-792                          */
+/* ------------------------------------------------------------------------ */
+/* scatsystem.h:715-721, scatsystem.c:2013-2037: the field of the coefficient
+ * vector cvf (fecv layout) at `where`, waves of kind btyp (QPMS_HANKEL_PLUS
+ * for a scattered field) at wavenumber k, evaluated on the GPU. Default VSWF
+ * normalisation; electric and magnetic waves up to the calculator's lMax (1..4). */
+ccart3_t qpms_scatsys_scattered_E(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, const cart3_t where);
+/* scatsystem.h:734-740, scatsystem.c:2039-2045 (finite systems only, aborts otherwise) */
+ccart3_t qpms_scatsysw_scattered_E(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where);
+/* scatsystem.h:750-756, scatsystem.c:2048-2103: the reference's translation-based
+ * variant. It translates with the vector particle - point, which gives the direct
+ * sum with electric coefficients times (-1)^(l+1) and magnetic ones times (-1)^l;
+ * that value is returned (QPMS_HIP_SCATTERED_E_ALT=direct returns the direct sum). */
+ccart3_t qpms_scatsys_scattered_E__alt(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, const cart3_t where);
+/* scatsystem.h:767-773, scatsystem.c:2105-2109 */
+ccart3_t qpms_scatsysw_scattered_E__alt(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where);
+/* Extension: qpms_scatsys_scattered_E at npts points in one launch (host arrays). */
+void qpms_hip_scatsys_scattered_E_points(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, size_t npts, const cart3_t *where, ccart3_t *out);
+/* Extension, device pointers on `stream`: particles at pos_dev[3 npart], coefficients
+ * cvf_dev[npart][2 nelem(lMax)] (complex; electric block, then magnetic, default
+ * order), points where_dev[3 npts]; out_dev[3 npts] complex (x, y, z). parity != 0
+ * multiplies electric coefficients by (-1)^(l+1), magnetic ones by (-1)^l. */
+int qpms_hip_scattered_E_dev(int lMax, size_t npart, const double *pos_dev, _Complex double k, qpms_bessel_t btyp,
+		const double *cvf_dev, size_t npts, const double *where_dev, double *out_dev, int parity, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Beyn contour eigensolver — beyn.h:11-126                                  */
 /* ------------------------------------------------------------------------ */
 typedef int (*beyn_function_M_t)(_Complex double *target_M, size_t m, _Complex double z, void *params);
@@ -666,6 +697,16 @@ int qpms_hip_solve_mixed_dev(size_t n, size_t nrhs, const _Complex double *A_dev
 beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_function_M_inv_Vhat_t M_inv_Vhat,
 		void *params, const beyn_contour_t *contour, double rank_tol, size_t rank_min_sel,
 		double res_tol, const _Complex double *Vhat);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement probes (no reference counterpart): the measured HBM bandwidth */
+/* and FP64 vector throughput that roofline fractions are compared with.     */
+/* ------------------------------------------------------------------------ */
+/* STREAM triad a = b + s c over n doubles (n even; 24 n bytes), device pointers. */
+int qpms_hip_probe_triad(size_t n, double *a, const double *b, const double *c, double s, void *stream);
+/* blocks x 256 lanes of 8 independent FP64 FMA chains for iters steps
+ * (16 iters flops per lane); out_dev holds blocks * 256 doubles. */
+int qpms_hip_probe_fma64(int blocks, int iters, double *out_dev, void *stream);
 
 #ifdef __cplusplus
 }

@@ -61,10 +61,12 @@ void or_sph_bessel(int J, int lmax, double complex xd, double complex *out) {
 		for (int l = 0; l <= lmax; ++l) out[l] = (J == 1) ? (l == 0 ? 1 : 0) : NAN + I * NAN;
 		return;
 	}
-	if (J >= 3 && cimagl(x) != 0) {
+	/* Hankel finite sums also for real arguments far above the order, where
+	 * the downward recurrence below would need O(|x|) steps; j = Re h, y = Im h. */
+	if ((J >= 3 && cimagl(x) != 0) || (cimagl(x) == 0 && ax > 1000 + 10 * lmax)) {
 		/* spherical Hankel functions directly, DLMF 10.49.6:
 		 * h^(1,2)_n(z) = (-+i)^{n+1} e^{+-iz}/z sum_k (+-i/(2z))^k (n+k)!/(k!(n-k)!) */
-		const ld sg = (J == 3) ? 1 : -1;
+		const ld sg = (J == 4) ? -1 : 1;
 		const ldc e = cexpl(sg * I * x) / x;
 		for (int n = 0; n <= lmax; ++n) {
 			ldc sum = 0, term = 1;
@@ -74,7 +76,12 @@ void or_sph_bessel(int J, int lmax, double complex xd, double complex *out) {
 			}
 			ldc ph = 1;
 			for (int q = 0; q <= n; ++q) ph *= -sg * I;
-			out[n] = (double complex)(ph * e * sum);
+			const ldc h = ph * e * sum;
+			if (cimagl(x) != 0) out[n] = (double complex)h;
+			else {
+				const double jr = (double)creall(h), yr = (double)(sg * cimagl(h));
+				out[n] = (J == 1) ? jr : (J == 2) ? yr : (J == 3) ? jr + I * yr : jr - I * yr;
+			}
 		}
 		return;
 	}

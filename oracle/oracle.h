@@ -92,4 +92,15 @@ void or_e32_combine(const or_calc *c, const double complex *sigma_total, int bqm
 void or_planewave(int lMax, int npart, const double *pos, const double *kvec, const double complex *E,
 		double complex *out);
 
+/* Field of the coefficient vector cvf (npart blocks of 2*nelem, default
+ * basis) at npts cartesian points: out[3*t + (x, y, z)], waves of kind J at
+ * wavenumber k (qpms_scatsys_scattered_E, scatsystem.c:2013-2037). */
+void or_scattered_E(int lMax, int npart, const double *pos, double complex k, int J, const double complex *cvf,
+		int npts, const double *where, double complex *out);
+/* The same sum as the reference's __alt variant computes it (scatsystem.c:
+ * 2048-2109): translation arrays with destination = regular electric dipoles
+ * at the particle, source = the point, times the dipoles' value at r = 0. */
+void or_scattered_E_alt(const or_calc *c, int npart, const double *pos, double complex k, int J,
+		const double complex *cvf, int npts, const double *where, double complex *out);
+
 #endif

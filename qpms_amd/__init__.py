@@ -86,6 +86,8 @@ def lib():
         L.qpms_hip_zgetrs_dev.argtypes = [sz, sz, vp, vp, vp, vp]
         L.qpms_hip_solve_mixed_dev.argtypes = [sz, sz, vp, vp, vp, dbl, i, vp]
         L.qpms_hip_version.restype = ctypes.c_char_p
+        L.qpms_hip_scattered_E_dev.argtypes = [i, sz, vp, c_cplx, i, vp, sz, vp, vp, i, vp]
+        L.qpms_hip_scattered_E_dev.restype = i
         _lib = L
     return _lib
 
@@ -194,6 +196,30 @@ class TransCalculator:
             lib().qpms_trans_calculator_free(self.c)
         except Exception:
             pass
+
+
+def scattered_E(lMax, pos, k, cvf, where, J=HANKEL_PLUS, parity=False):
+    """Field of the coefficient vectors cvf (npart x 2 nelem, default basis:
+    electric block, then magnetic) at the points where (npts x 3), waves of
+    kind J at wavenumber k: complex (npts, 3) cartesian components
+    (qpms_hip_scattered_E_dev)."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    where = np.ascontiguousarray(where, np.float64).reshape(-1, 3)
+    cvf = np.ascontiguousarray(cvf, np.complex128).reshape(len(pos), -1)
+    assert cvf.shape[1] == 2 * lMax * (lMax + 2)
+    if len(where) == 0:   # nothing to launch; the entry point is still called with no buffers
+        assert lib().qpms_hip_scattered_E_dev(lMax, len(pos), None, c_cplx.of(k), J, None, 0, None, None,
+                                              int(parity), None) == 0
+        return np.zeros((0, 3), np.complex128)
+    dp, dc, dw = DeviceBuffer.from_array(pos), DeviceBuffer.from_array(cvf), DeviceBuffer.from_array(where)
+    do = DeviceBuffer(len(where) * 48)
+    rc = lib().qpms_hip_scattered_E_dev(lMax, len(pos), dp.ptr, c_cplx.of(k), J, dc.ptr, len(where), dw.ptr, do.ptr,
+                                        int(parity), None)
+    assert rc == 0
+    out = do.to_array(np.complex128, (len(where), 3))
+    for b in (dp, dc, dw, do):
+        b.free()
+    return out
 
 
 def zgetrf(A):

@@ -10,6 +10,14 @@
 //                per output row.
 //  k_cdot        -Re(a^H f) partial sums for the extinction cross-section
 //                (misc/finiterectlat-scatter.py:108), one block per batch item.
+//  k_scattered_e field of VSWF coefficient vectors at points: per (point,
+//                particle) the VSWFs of qpms_vswf_fill_csph (vswf.c:230-288)
+//                at k (point - particle), contracted with the particle's
+//                coefficients (vswf.c:611-655), turned to cartesian
+//                (vectors.h:272-291) and summed over particles
+//                (scatsystem.c:2013-2037). One thread per point, or one wave
+//                per point with the lanes splitting the particles when points
+//                are few and particles many.
 #include <hip/hip_runtime.h>
 #include "dev_specfunc.hpp"
 #include "qpms_internal.h"
@@ -132,6 +140,113 @@ __global__ void k_neg_re_cdot(long n, const v2d *__restrict__ a, long sa, const 
   if (threadIdx.x == 0) out[blockIdx.x] = -red[0];
 }
 
+
+// One particle's field at offset (dx, dy, dz) from it, added to E (cartesian).
+// c: the particle's 2 NE coefficients (electric block, then magnetic);
+// parity: multiply electric coefficients by (-1)^(l+1), magnetic by (-1)^l.
+template <int LMAX>
+__device__ __forceinline__ void field_of_particle(double dx, double dy, double dz, cplx k, int kind,
+                                                  const v2d *__restrict__ c, bool parity, cplx *E) {
+  constexpr int NE = LMAX * (LMAX + 2);
+  constexpr double PI_ = 3.14159265358979323846;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double theta = r != 0.0 ? acos(dz / r) : PI_ / 2;   // cart2sph (vectors.h:134-140)
+  const double phi = atan2(dy, dx);
+  const double ct = cos(theta), st = sin(theta), cf = cos(phi), sf = sin(phi);
+  const cplx kr = k * r;
+  cplx z[LMAX + 1];
+  qsf::sph_bessel_fill<LMAX>(kind, kr, z);
+  const bool origin = (kr.re == 0.0 && kr.im == 0.0);
+  const cplx krinv = origin ? mk(0.0, 0.0) : cinv(kr);
+  const bool pole = (fabs(ct) == 1.0);   // legendre.c:65-80
+  double P[qsf::leg_n(LMAX)];
+  qsf::legendre_cs<LMAX>(ct, st, P);
+  cplx Er = mk(0.0, 0.0), Et = mk(0.0, 0.0), Ef = mk(0.0, 0.0);
+#pragma unroll
+  for (int l = 1; l <= LMAX; ++l) {
+    // z_l / kr (1/3 for the regular l = 1 wave at the origin) and z_{l-1} - l z_l / kr
+    const cplx besfac = origin ? mk(l == 1 ? 1.0 / 3.0 : 0.0, 0.0) : z[l] * krinv;
+    const cplx besder = z[l - 1] - (double)l * besfac;
+    const double pe = (parity && !(l & 1)) ? -1.0 : 1.0;   // electric: (-1)^(l+1)
+    const double pm = (parity && (l & 1)) ? -1.0 : 1.0;    // magnetic: (-1)^l
+    cplx em = mk(1.0, 0.0);   // e^{i m phi}
+#pragma unroll
+    for (int m = 0; m <= l; ++m) {
+      if (m > 0) em = em * mk(cf, sf);
+      // normalised leg, pi, tau for +m (legendre.c:54-110, default normalisation)
+      double leg, pi, tau;
+      if (pole) {
+        const double fl = 0.25 * sqrt((2 * l + 1) / PI_);
+        const double lpar = (l & 1) ? -1.0 : 1.0;
+        leg = (m == 0) ? ((l & 1) ? ct : 1.0) * sqrt((2 * l + 1) / (4 * PI_ * l * (l + 1))) : 0.0;
+        pi = (m == 1) ? ((ct > 0) ? -1.0 : lpar) * fl : 0.0;     // -(...) * fl * csphase, csphase = -1
+        tau = (m == 1) ? -((ct > 0) ? 1.0 : lpar) * fl : 0.0;
+      } else {
+        double fr = 1.0;
+#pragma unroll
+        for (int t = l - m + 1; t <= l + m; ++t) fr *= t;
+        const double nlm = sqrt((2 * l + 1) / (4 * PI_ * fr)) / sqrt((double)(l * (l + 1)));
+        const double Plm = P[qsf::leg_idx(l, m)];
+        const double Pl1 = (l - 1 >= m) ? P[qsf::leg_idx(l - 1, m)] : 0.0;
+        const double dP = ((l + m) * Pl1 - l * ct * Plm) / ((1.0 - ct) * (1.0 + ct));   // DLMF 14.10.5
+        leg = nlm * Plm;
+        pi = m / st * leg;
+        tau = -st * nlm * dP;
+      }
+#pragma unroll
+      for (int sgn = 0; sgn < (m == 0 ? 1 : 2); ++sgn) {
+        // -m: leg and tau carry (-1)^m, pi (-1)^(m+1); e^{-i m phi} = conj
+        const int mm = sgn ? -m : m;
+        const double fm = (sgn && (m & 1)) ? -1.0 : 1.0;
+        const double lg = fm * leg, ta = fm * tau, p_ = sgn ? -fm * pi : pi;
+        const cplx e = sgn ? conj(em) : em;
+        const int y = l * (l + 1) + mm - 1;
+        const v2d cn = c[y], cm = c[NE + y];
+        const cplx cN = mk(pe * cn.x, pe * cn.y), cM = mk(pm * cm.x, pm * cm.y);
+        const cplx be = besfac * e, bde = besder * e, ze = z[l] * e;
+        const cplx nbde = cN * bde, mze = cM * ze;
+        Er = Er + ((double)(l * (l + 1)) * lg) * (cN * be);
+        // N: (l(l+1) leg z/kr e, tau besder e, pi besder i e); M: (0, pi z i e, -tau z e)
+        Et = Et + ta * nbde + p_ * mk(-mze.im, mze.re);
+        Ef = Ef + p_ * mk(-nbde.im, nbde.re) - ta * mze;
+      }
+    }
+  }
+  E[0] = E[0] + (st * cf) * Er + (ct * cf) * Et - sf * Ef;
+  E[1] = E[1] + (st * sf) * Er + (ct * sf) * Et + cf * Ef;
+  E[2] = E[2] + ct * Er - st * Et;
+}
+
+template <int LMAX, bool WAVE>
+__global__ __launch_bounds__(256) void k_scattered_e(int npts, const double *__restrict__ where, int npart,
+                                                     const double *__restrict__ pos, double k_re, double k_im,
+                                                     int kind, const v2d *__restrict__ cvf, int parity,
+                                                     v2d *__restrict__ out) {
+  constexpr int NB = 2 * LMAX * (LMAX + 2);
+  const int lane = threadIdx.x & 63;
+  const int t = WAVE ? (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6))
+                     : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= npts) return;
+  const double wx = where[3 * t], wy = where[3 * t + 1], wz = where[3 * t + 2];
+  const cplx k = mk(k_re, k_im);
+  cplx E[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+  for (int p = WAVE ? lane : 0; p < npart; p += WAVE ? 64 : 1)
+    field_of_particle<LMAX>(wx - pos[3 * p], wy - pos[3 * p + 1], wz - pos[3 * p + 2], k, kind,
+                            cvf + (size_t)p * NB, parity != 0, E);
+  if (WAVE) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        E[q].re += __shfl_xor(E[q].re, o);
+        E[q].im += __shfl_xor(E[q].im, o);
+      }
+    if (lane != 0) return;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) out[3 * (size_t)t + q] = v2d{E[q].re, E[q].im};
+}
+
 }  // namespace
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
@@ -172,6 +287,42 @@ extern "C" int qh_launch_neg_re_cdot(int batch, long n, const double *a_dev, lon
   hipStream_t s = (hipStream_t)qh_stream(stream);
   hipLaunchKernelGGL(k_neg_re_cdot, dim3(batch), dim3(256), 0, s, n, (const v2d *)a_dev, sa, (const v2d *)f_dev, sf,
                      out_dev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Points at where[3 npts], particles at pos[3 npart], coefficients cvf[npart][2 nelem(lMax)];
+// out[3 npts] complex. parity != 0 multiplies electric coefficients by (-1)^(l+1)
+// and magnetic ones by (-1)^l first.
+extern "C" int qh_launch_scattered_e(int lMax, size_t npts, const double *where_dev, size_t npart,
+                                     const double *pos_dev, double k_re, double k_im, int kind,
+                                     const double *cvf_dev, int parity, double *out_dev, void *stream) {
+  if (npts == 0) return 0;
+  if (npts > (size_t)INT32_MAX || npart > (size_t)INT32_MAX)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "too many points (%zu) or particles (%zu)", npts, npart);
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  // a wave per point when points are too few to fill the device thread-per-point
+  const bool wave = npart >= 32 && npts < 262144;
+  const dim3 grid((unsigned)(wave ? (npts + 3) / 4 : (npts + 255) / 256)), block(256);
+  const v2d *c = (const v2d *)cvf_dev;
+  v2d *o = (v2d *)out_dev;
+#define QH_SCE(LM)                                                                                              \
+  do {                                                                                                          \
+    if (wave)                                                                                                   \
+      hipLaunchKernelGGL((k_scattered_e<LM, true>), grid, block, 0, s, (int)npts, where_dev, (int)npart, pos_dev, \
+                         k_re, k_im, kind, c, parity, o);                                                       \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_scattered_e<LM, false>), grid, block, 0, s, (int)npts, where_dev, (int)npart,       \
+                         pos_dev, k_re, k_im, kind, c, parity, o);                                              \
+  } while (0)
+  switch (lMax) {
+    case 1: QH_SCE(1); break;
+    case 2: QH_SCE(2); break;
+    case 3: QH_SCE(3); break;
+    case 4: QH_SCE(4); break;
+    default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", lMax);
+  }
+#undef QH_SCE
   HIPCHK(hipGetLastError());
   return 0;
 }

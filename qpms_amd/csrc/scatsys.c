@@ -695,3 +695,102 @@ complex double *qpms_scatsysw_apply_Tmatrices_full(complex double *target_full, 
 	}
 	return target_full;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Scattered field at points (scatsystem.c:2013-2109)                        */
+/* ------------------------------------------------------------------------ */
+int qh_launch_scattered_e(int lMax, size_t npts, const double *where_dev, size_t npart, const double *pos_dev,
+		double k_re, double k_im, int kind, const double *cvf_dev, int parity, double *out_dev, void *stream);
+
+/* The coefficient vector in the kernel's layout: npart blocks of 2 nelem of
+ * the calculator's lMax (electric, then magnetic), zeros where a particle's
+ * basis has no entry. */
+static complex double *qh_pad_cvf(const qpms_scatsys_t *ss, const complex double *cvf) {
+	QH_ENSURE(ss->c->normalisation == QPMS_NORMALISATION_DEFAULT,
+		"the scattered field is implemented for the default VSWF normalisation only");
+	const size_t ne = ss->c->nelem, np = ss->p_count;
+	complex double *c;
+	QH_CALLOC(c, np * 2 * ne, sizeof(complex double));
+	for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi) {
+		const qpms_vswf_set_spec_t *b = qh_bspec_pi(ss, pi);
+		QH_ENSURE(b->norm == QPMS_NORMALISATION_DEFAULT, "the scattered field is implemented for the default VSWF normalisation only");
+		for (size_t i = 0; i < b->n; ++i) {
+			const qpms_uvswfi_t u = b->ilist[i];
+			const int t = (int)(u & 3);
+			const long y = (long)(u / 4) - 1;
+			QH_ENSURE((t == QPMS_VSWF_ELECTRIC || t == QPMS_VSWF_MAGNETIC) && y >= 0 && (size_t)y < ne,
+				"scattered field: basis entry %llu is not an electric or magnetic wave of degree <= %d",
+				(unsigned long long)u, (int)ss->c->lMax);
+			c[pi * 2 * ne + (t == QPMS_VSWF_ELECTRIC ? 0 : ne) + (size_t)y] = cvf[ss->fecv_pstarts[pi] + i];
+		}
+	}
+	return c;
+}
+
+/* The reference's __alt variants translate with the vector particle - point
+ * (destination = the dipoles at the particle, source = the point), which is
+ * the direct sum with electric coefficients times (-1)^(l+1) and magnetic ones
+ * times (-1)^l. QPMS_HIP_SCATTERED_E_ALT=direct gives the direct sum instead. */
+static int qh_alt_parity(void) {
+	const char *e = getenv("QPMS_HIP_SCATTERED_E_ALT");
+	if (!e || !strcmp(e, "reference")) return 1;
+	QH_ENSURE(!strcmp(e, "direct"), "QPMS_HIP_SCATTERED_E_ALT must be 'reference' or 'direct', not '%s'", e);
+	return 0;
+}
+
+int qpms_hip_scattered_E_dev(int lMax, size_t npart, const double *pos_dev, _Complex double k, qpms_bessel_t btyp,
+		const double *cvf_dev, size_t npts, const double *where_dev, double *out_dev, int parity, void *stream) {
+	QH_ENSURE(lMax >= 1 && lMax <= 4, "lMax %d outside 1..4", lMax);
+	QH_ENSURE(btyp >= QPMS_BESSEL_REGULAR && btyp <= QPMS_HANKEL_MINUS, "invalid Bessel type %d", (int)btyp);
+	return qh_launch_scattered_e(lMax, npts, where_dev, npart, pos_dev, creal(k), cimag(k), (int)btyp, cvf_dev,
+		parity, out_dev, stream);
+}
+
+static void qh_scattered_E_points(const qpms_scatsys_t *ss, qpms_bessel_t btyp, complex double k,
+		const complex double *cvf, size_t npts, const cart3_t *where, ccart3_t *out, int parity) {
+	if (npts == 0) return;
+	complex double *c = qh_pad_cvf(ss, cvf);
+	const size_t np = ss->p_count, nb = 2 * ss->c->nelem;
+	double *dpos = qh_upload_positions(ss);
+	double *dc = qh_dmalloc(np * nb * sizeof(complex double));
+	double *dw = qh_dmalloc(npts * sizeof(cart3_t));
+	double *dout = qh_dmalloc(npts * sizeof(ccart3_t));
+	qh_h2d(dc, c, np * nb * sizeof(complex double), NULL);
+	qh_h2d(dw, where, npts * sizeof(cart3_t), NULL);
+	qpms_hip_scattered_E_dev((int)ss->c->lMax, np, dpos, k, btyp, dc, npts, dw, dout, parity, NULL);
+	qh_d2h(out, dout, npts * sizeof(ccart3_t), NULL);
+	qh_sync(NULL);
+	qh_dfree(dout); qh_dfree(dw); qh_dfree(dc); qh_dfree(dpos);
+	free(c);
+}
+
+void qpms_hip_scatsys_scattered_E_points(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, size_t npts, const cart3_t *where, ccart3_t *out) {
+	qh_scattered_E_points(ss, btyp, k, cvf, npts, where, out, 0);
+}
+
+ccart3_t qpms_scatsys_scattered_E(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, const cart3_t where) {
+	ccart3_t r;
+	qh_scattered_E_points(ss, btyp, k, cvf, 1, &where, &r, 0);
+	return r;
+}
+
+ccart3_t qpms_scatsysw_scattered_E(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where) {
+	QH_ENSURE(ssw->ss->lattice_dimension == 0, "qpms_scatsysw_scattered_E(): applicable only to nonperiodic systems.");
+	return qpms_scatsys_scattered_E(ssw->ss, btyp, ssw->wavenumber, cvf, where);
+}
+
+ccart3_t qpms_scatsys_scattered_E__alt(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
+		const _Complex double *cvf, const cart3_t where) {
+	QH_ENSURE(ss->lattice_dimension == 0, "qpms_scatsys_scattered_E__alt(): applicable only to nonperiodic systems.");
+	ccart3_t r;
+	qh_scattered_E_points(ss, btyp, k, cvf, 1, &where, &r, qh_alt_parity());
+	return r;
+}
+
+ccart3_t qpms_scatsysw_scattered_E__alt(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where) {
+	return qpms_scatsys_scattered_E__alt(ssw->ss, btyp, ssw->wavenumber, cvf, where);
+}

@@ -4,10 +4,14 @@
  * W, M = I - T W, LU, incident plane wave, T a, solve, extinction.
  *
  * usage: ss_finite OUT nx ny pitch radius epsin_re epsin_im eps_bg eV lMax
- *                  kx ky kz Ex_re Ex_im Ey_re Ey_im Ez_re Ez_im
+ *                  kx ky kz Ex_re Ex_im Ey_re Ey_im Ez_re Ez_im [POINTS]
  * Particles at ((i - (nx-1)/2) pitch, (j - (ny-1)/2) pitch, 0). Writes to
  * OUT: int64 L, then complex128 W[L*L], M[L*L], a[L], Ta[L], f[L], T diag
- * [nb], and float64 sigma_ext, wavenumber (re, im). */
+ * [nb], and float64 sigma_ext, wavenumber (re, im). With POINTS (float64
+ * x, y, z per point) it then appends int64 npts and the scattered field of f
+ * (complex128 x, y, z per point) from qpms_scatsys_scattered_E,
+ * qpms_scatsysw_scattered_E, qpms_scatsysw_scattered_E__alt and
+ * qpms_hip_scatsys_scattered_E_points, in that order. */
 #include <complex.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -17,9 +21,9 @@
 #define EV_RAD_PER_S (1.602176634e-19 / 1.054571817e-34)
 
 int main(int argc, char **argv) {
-	if (argc != 20) {
+	if (argc != 20 && argc != 21) {
 		fprintf(stderr, "usage: %s OUT nx ny pitch radius epsin_re epsin_im eps_bg eV lMax kx ky kz "
-			"Ex_re Ex_im Ey_re Ey_im Ez_re Ez_im\n", argv[0]);
+			"Ex_re Ex_im Ey_re Ey_im Ez_re Ez_im [POINTS]\n", argv[0]);
 		return 2;
 	}
 	const char *out = argv[1];
@@ -94,6 +98,26 @@ int main(int argc, char **argv) {
 		&& fwrite(a, sizeof *a, L, fo) == L && fwrite(Ta, sizeof *Ta, L, fo) == L && fwrite(f, sizeof *f, L, fo) == L
 		&& fwrite(td, sizeof *td, nb, fo) == nb && fwrite(&sigma_ext, sizeof sigma_ext, 1, fo) == 1
 		&& fwrite(kk, sizeof *kk, 2, fo) == 2;
+	if (ok && argc == 21) {
+		FILE *fp = fopen(argv[20], "rb");
+		if (!fp) { perror(argv[20]); return 4; }
+		fseek(fp, 0, SEEK_END);
+		const long long npts = ftell(fp) / (long long)sizeof(cart3_t);
+		fseek(fp, 0, SEEK_SET);
+		cart3_t *w = malloc((size_t)npts * sizeof *w);
+		ccart3_t *e = malloc((size_t)npts * 4 * sizeof *e);
+		ok = fread(w, sizeof *w, (size_t)npts, fp) == (size_t)npts;
+		fclose(fp);
+		for (long long t = 0; ok && t < npts; ++t) {
+			e[t] = qpms_scatsys_scattered_E(ss, QPMS_HANKEL_PLUS, k, f, w[t]);
+			e[npts + t] = qpms_scatsysw_scattered_E(ssw, QPMS_HANKEL_PLUS, f, w[t]);
+			e[2 * npts + t] = qpms_scatsysw_scattered_E__alt(ssw, QPMS_HANKEL_PLUS, f, w[t]);
+		}
+		if (ok) qpms_hip_scatsys_scattered_E_points(ss, QPMS_HANKEL_PLUS, k, f, (size_t)npts, w, e + 3 * npts);
+		ok = ok && fwrite(&npts, sizeof npts, 1, fo) == 1
+			&& fwrite(e, sizeof *e, (size_t)npts * 4, fo) == (size_t)npts * 4;
+		free(w); free(e);
+	}
 	fclose(fo);
 	printf("L=%zu sigma_ext=%.17g\n", L, sigma_ext);
 	free(td); free(a); free(Ta); free(f); free(W); free(M);

@@ -54,6 +54,8 @@ def oracle():
         L.or_sph_bessel.argtypes = [i32, i32, c_cplx, vp]
         L.or_calc_multipliers.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         L.or_set_threads.argtypes = [i32]
+        L.or_scattered_E.argtypes = [i32, i32, vp, c_cplx, i32, vp, i32, vp, vp]
+        L.or_scattered_E_alt.argtypes = [vp, i32, vp, c_cplx, i32, vp, i32, vp, vp]
         _o = L
     return _o
 
@@ -195,3 +197,30 @@ def finite_cluster(pos, lMax, radius, eps_in, eps_bg, eV, kdir, E):
     f = zgetrs(LU, ipiv, Ta)
     sigma = -np.vdot(a, f).real / (k * k.conjugate()).real
     return dict(W=W, M=M, a=a, Ta=Ta, f=f, tdiag=tdiag, k=k, sigma_ext=sigma)
+
+
+def scattered_E(lMax, pos, k, cvf, where, J=3):
+    """Field of the coefficient vector cvf (npart x 2 nelem, default basis) at
+    the points where (npts x 3): complex (npts, 3) cartesian components."""
+    o = oracle()
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    where = np.ascontiguousarray(where, np.float64).reshape(-1, 3)
+    cvf = np.ascontiguousarray(cvf, np.complex128)
+    out = np.empty((len(where), 3), np.complex128)
+    o.or_scattered_E(lMax, len(pos), pos.ctypes.data, c_cplx.of(k), J, cvf.ctypes.data, len(where),
+                     where.ctypes.data, out.ctypes.data)
+    return out
+
+
+def scattered_E_alt(lMax, pos, k, cvf, where, J=3):
+    """The same field through translation arrays to regular dipoles, as the
+    reference's __alt variant forms it."""
+    o = oracle()
+    c = OracleCalc(lMax)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    where = np.ascontiguousarray(where, np.float64).reshape(-1, 3)
+    cvf = np.ascontiguousarray(cvf, np.complex128)
+    out = np.empty((len(where), 3), np.complex128)
+    o.or_scattered_E_alt(c.c, len(pos), pos.ctypes.data, c_cplx.of(k), J, cvf.ctypes.data, len(where),
+                         where.ctypes.data, out.ctypes.data)
+    return out

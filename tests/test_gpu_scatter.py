@@ -66,23 +66,34 @@ def test_incfield_planewave(gpu, lMax, case):
     L.qpms_vswf_set_spec_free(bspec)
 
 
-def run_driver(tmp_path, nx, ny, pitch, radius, eps_in, eps_bg, eV, lMax, kd, E):
+def run_driver(tmp_path, nx, ny, pitch, radius, eps_in, eps_bg, eV, lMax, kd, E, points=None):
     out = tmp_path / "ss.bin"
     args = [DRIVER, str(out), str(nx), str(ny), repr(pitch), repr(radius), repr(eps_in.real), repr(eps_in.imag),
             repr(eps_bg), repr(eV), str(lMax)] + [repr(float(x)) for x in kd]
     for e in E:
         args += [repr(complex(e).real), repr(complex(e).imag)]
+    if points is not None:
+        pf = tmp_path / "points.bin"
+        pf.write_bytes(np.ascontiguousarray(points, np.float64).tobytes())
+        args.append(str(pf))
     subprocess.run(args, check=True, timeout=120)
     raw = out.read_bytes()
     L = int(np.frombuffer(raw[:8], np.int64)[0])
     nb = L // (nx * ny)
-    c = np.frombuffer(raw[8:8 + 16 * (2 * L * L + 3 * L + nb)], np.complex128)
-    tail = np.frombuffer(raw[8 + 16 * (2 * L * L + 3 * L + nb):], np.float64)
+    nc = 2 * L * L + 3 * L + nb
+    c = np.frombuffer(raw[8:8 + 16 * nc], np.complex128)
+    tail = np.frombuffer(raw[8 + 16 * nc:8 + 16 * nc + 24], np.float64)
     W = c[:L * L].reshape(L, L)
     M = c[L * L:2 * L * L].reshape(L, L)
     a, Ta, f = c[2 * L * L:2 * L * L + L], c[2 * L * L + L:2 * L * L + 2 * L], c[2 * L * L + 2 * L:2 * L * L + 3 * L]
     td = c[2 * L * L + 3 * L:]
-    return dict(W=W, M=M, a=a, Ta=Ta, f=f, tdiag=td, sigma_ext=tail[0], k=complex(tail[1], tail[2]))
+    r = dict(W=W, M=M, a=a, Ta=Ta, f=f, tdiag=td, sigma_ext=tail[0], k=complex(tail[1], tail[2]))
+    if points is not None:
+        o = 8 + 16 * nc + 24
+        npts = int(np.frombuffer(raw[o:o + 8], np.int64)[0])
+        e = np.frombuffer(raw[o + 8:], np.complex128).reshape(4, npts, 3)
+        r.update(E_single=e[0], E_w=e[1], E_alt=e[2], E_points=e[3])
+    return r
 
 
 def rel(a, b):
@@ -109,3 +120,38 @@ def test_finite_cluster_abi(gpu, tmp_path, shape, lMax, kd, E):
     assert rel(g["Ta"], o["Ta"]) < 1e-12
     assert rel(g["f"], o["f"]) < 1e-9
     assert abs(g["sigma_ext"] - o["sigma_ext"]) < 1e-9 * abs(o["sigma_ext"])
+
+
+def _field_points(pos, rng, n=40):
+    """Points near and between the particles, on a particle's z axis (both
+    poles) and far away."""
+    lo, hi = pos.min(axis=0) - 300e-9, pos.max(axis=0) + 300e-9
+    pts = [lo + (hi - lo) * rng.uniform(size=3) for _ in range(n)]
+    pts += [pos[0] + [0, 0, 120e-9], pos[-1] - [0, 0, 90e-9], pos[0] + [70e-9, 0, 0]]
+    pts += [pos.mean(axis=0) + 1e-3 * np.array([0.3, -0.5, 0.8]), pos.mean(axis=0) + [0, 0, 2e-6]]
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("shape,lMax,kd,E", [
+    ((4, 4), 2, (0, 0, 1.0), (1, 0, 0)),
+    ((3, 2), 3, (0.3, 0.4, np.sqrt(0.75)), (0.8, -0.6, 0)),
+    ((2, 2), 4, (0, 0, 1.0), (0, 1, 0)),
+])
+def test_scattered_field_abi(gpu, tmp_path, shape, lMax, kd, E):
+    """qpms_scatsys_scattered_E and its variants through the C ABI, on the
+    solved cluster's own f, against the oracle's field of that f (1e-10
+    relative to the largest field in the sample)."""
+    nx, ny = shape
+    pitch, radius, eps_in, eps_bg, eV = 500e-9, 50e-9, -12 + 1j, 2.31, 2.0
+    pos = np.array([[(i - 0.5 * (nx - 1)) * pitch, (j - 0.5 * (ny - 1)) * pitch, 0.0]
+                    for i in range(nx) for j in range(ny)])
+    pts = _field_points(pos, np.random.default_rng(lMax))
+    g = run_driver(tmp_path, nx, ny, pitch, radius, eps_in, eps_bg, eV, lMax, kd, E, points=pts)
+    ref = of.scattered_E(lMax, pos, g["k"], g["f"], pts)
+    alt = of.scattered_E_alt(lMax, pos, g["k"], g["f"], pts)
+    near = slice(0, len(pts) - 2)   # far points compared separately (1/r smaller)
+    for key in ("E_single", "E_w", "E_points"):
+        assert rel(g[key][near], ref[near]) < 1e-10, key
+        for t in (-2, -1):
+            assert np.max(np.abs(g[key][t] - ref[t])) < 1e-10 * np.max(np.abs(ref[t])), (key, t)
+    assert rel(g["E_alt"][near], alt[near]) < 1e-10

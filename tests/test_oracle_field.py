@@ -11,6 +11,19 @@ mathematics fixes, independently of the oracle's own formulas:
 * the extinction cross-section of one sphere through the full pipeline equals
   the Mie closed form (2 pi / k^2) sum (2n+1) Re(a_n + b_n), for any
   incidence direction.
+
+The field of a coefficient vector at points (the scattered-field sum):
+
+* regular waves with plane-wave coefficients reproduce the plane wave near
+  the expansion point, to the truncation error (k d)^lMax;
+* the reference's translation-based variant, restated with the translation
+  arrays (pinned above), equals the direct VSWF sum of the coefficients times
+  (-1)^(l+1) (electric) and (-1)^l (magnetic): the translation runs from the
+  particle to the point with the vector reversed;
+* far field: the forward amplitude gives the extinction cross-section of the
+  coefficient formula (optical theorem), and the integral of |E_s|^2 over a
+  far sphere gives the scattering cross-section, equal to the extinction for
+  lossless spheres.
 """
 import numpy as np
 import pytest
@@ -117,3 +130,91 @@ def test_cluster_optical_theorem_lossless():
     f = r["f"]
     sca = np.vdot(f, (Wreg + np.eye(len(f))) @ f).real / r["k"].real ** 2
     assert abs(sca - r["sigma_ext"]) < 1e-6 * abs(r["sigma_ext"])
+
+
+@pytest.mark.parametrize("lMax", [2, 3, 4])
+def test_regular_field_is_the_plane_wave(lMax):
+    rng = np.random.default_rng(lMax)
+    k = 1.1e7
+    kd, E = _dir(*DIRS[2])
+    pos = np.array([[10e-9, -20e-9, 5e-9]])
+    a = of.planewave(lMax, pos, k * kd, E)
+    for d in (3e-9, 3e-10):
+        u = rng.normal(size=(8, 3))
+        pts = pos + d * u / np.linalg.norm(u, axis=1)[:, None]
+        got = of.scattered_E(lMax, pos, k, a.ravel(), pts, J=1)
+        ref = np.exp(1j * k * (pts @ kd))[:, None] * E[None, :]
+        assert np.max(np.abs(got - ref)) < (k * d) ** lMax * np.max(np.abs(ref))
+    # the expansion point itself (kr = 0: only the l = 1 regular waves survive)
+    got = of.scattered_E(lMax, pos, k, a.ravel(), pos, J=1)
+    ref = np.exp(1j * k * (pos @ kd))[:, None] * E[None, :]
+    assert np.max(np.abs(got - ref)) < 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("J", [1, 3])
+@pytest.mark.parametrize("lMax", [1, 2, 4])
+def test_alt_variant_is_the_parity_reversed_direct_sum(lMax, J):
+    rng = np.random.default_rng(10 * lMax + J)
+    ne = lMax * (lMax + 2)
+    k = 1.2e7 * (1 + 0.05j)
+    pos = rng.normal(size=(3, 3)) * 300e-9
+    cv = rng.normal(size=(3, 2 * ne)) + 1j * rng.normal(size=(3, 2 * ne))
+    pts = np.concatenate([rng.normal(size=(5, 3)) * 600e-9, pos[:1] + [[0, 0, 150e-9]]])   # incl. the z axis
+    alt = of.scattered_E_alt(lMax, pos, k, cv.ravel(), pts, J=J)
+    ls = np.concatenate([[l] * (2 * l + 1) for l in range(1, lMax + 1)])
+    cvp = cv.copy()
+    cvp[:, :ne] *= (-1.0) ** (ls + 1)
+    cvp[:, ne:] *= (-1.0) ** ls
+    direct = of.scattered_E(lMax, pos, k, cvp.ravel(), pts, J=J)
+    assert np.max(np.abs(direct - alt)) < 1e-12 * np.max(np.abs(alt))
+    # and not the plain direct sum: the variant is not a rounding-level restatement of it
+    plain = of.scattered_E(lMax, pos, k, cv.ravel(), pts, J=J)
+    assert np.max(np.abs(plain - alt)) > 1e-3 * np.max(np.abs(alt))
+
+
+def _far_sphere(nt=40, nphi=80):
+    x, w = np.polynomial.legendre.leggauss(nt)
+    th = np.arccos(x)
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], axis=-1).reshape(-1, 3)
+    wts = (w[:, None] * np.full(nphi, 2 * np.pi / nphi)[None, :]).ravel()
+    return dirs, wts
+
+
+@pytest.mark.parametrize("case", [0, 2, 3])
+def test_far_field_optical_theorem_single_sphere(case):
+    kd, E = _dir(*DIRS[case])
+    E = E / np.sqrt(np.vdot(E, E).real)
+    lMax, radius, eV = 4, 60e-9, 2.0
+    p = np.array([[10e-9, 5e-9, -3e-9]])
+    for eps_in in (-12 + 1j, 6.0):
+        r = of.finite_cluster(p, lMax, radius, eps_in, 2.31, eV, kd, E)
+        k = r["k"].real
+        R = 1e6 / k
+        fwd = p + R * kd
+        Es = of.scattered_E(lMax, p, k, r["f"], fwd)[0]
+        X = -1j * k * R * Es * np.exp(-1j * k * (fwd[0] @ kd))
+        ext = 4 * np.pi / k ** 2 * np.vdot(E, X).real
+        assert abs(ext - r["sigma_ext"]) < 1e-5 * abs(r["sigma_ext"])
+        if eps_in == 6.0:
+            dirs, wts = _far_sphere()
+            Ef = of.scattered_E(lMax, p, k, r["f"], p + R * dirs)
+            sca = np.sum(wts * np.sum(np.abs(Ef) ** 2, axis=1)) * R ** 2
+            assert abs(sca - r["sigma_ext"]) < 1e-5 * abs(r["sigma_ext"])
+
+
+def test_far_field_cluster_lossless():
+    pos = np.array([[i * 300e-9, j * 300e-9, 0] for i in range(2) for j in range(2)])
+    kd, E = np.array([0.0, 0.6, 0.8]), np.array([1.0, 0, 0], complex)
+    r = of.finite_cluster(pos, 3, 50e-9, 6.0, 2.31, 2.0, kd, E)
+    k = r["k"].real
+    R = 1e9 / k   # far enough that the cluster's extent (k d^2 / R) does not shift the phase
+    X = -1j * k * R * of.scattered_E(3, pos, k, r["f"], [R * kd])[0] * np.exp(-1j * k * R)
+    ext = 4 * np.pi / k ** 2 * np.vdot(E, X).real
+    assert abs(ext - r["sigma_ext"]) < 1e-5 * abs(r["sigma_ext"])
+    R = 1e6 / k
+    dirs, wts = _far_sphere()
+    Ef = of.scattered_E(3, pos, k, r["f"], R * dirs)
+    sca = np.sum(wts * np.sum(np.abs(Ef) ** 2, axis=1)) * R ** 2
+    assert abs(sca - r["sigma_ext"]) < 1e-5 * abs(r["sigma_ext"])

@@ -216,3 +216,19 @@ def test_regular_operator_reciprocity():
     keep = [i for i in range(ne) if y2mn(i)[1] <= 2]
     keep = keep + [ne + i for i in keep]
     assert np.max(np.abs(P[np.ix_(keep, keep)] - np.eye(len(keep)))) < 1e-6
+
+
+def test_bessel_large_real_argument_vs_mpmath():
+    """Real arguments far above the order take the Hankel finite sums
+    (DLMF 10.49.6) instead of the downward recurrence."""
+    import mpmath
+    mpmath.mp.dps = 40
+    for x in (1100.5, 5000.25, 1.0e6 + 0.125):
+        j, y, h, hm = (of.sph_bessel(J, 9, x) for J in (1, 2, 3, 4))
+        for n in range(10):
+            f = mpmath.sqrt(mpmath.pi / (2 * mpmath.mpf(x)))
+            jr = float(f * mpmath.besselj(n + 0.5, x))
+            yr = float(f * mpmath.bessely(n + 0.5, x))
+            s = np.hypot(jr, yr)
+            assert abs(j[n] - jr) < 1e-13 * s and abs(y[n] - yr) < 1e-13 * s, (x, n)
+            assert abs(h[n] - (jr + 1j * yr)) < 1e-13 * s and abs(hm[n] - (jr - 1j * yr)) < 1e-13 * s, (x, n)
