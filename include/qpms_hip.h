@@ -552,6 +552,11 @@ ccart3_t qpms_scatsys_scattered_E__alt(const qpms_scatsys_t *ss, qpms_bessel_t b
 /* scatsystem.h:767-773, scatsystem.c:2105-2109 */
 ccart3_t qpms_scatsysw_scattered_E__alt(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
 		const _Complex double *cvf, const cart3_t where);
+/* scatsystem.h:786-792, scatsystem.c:2113-2188: 2D-periodic systems, from the
+ * lattice-summed translation arrays (QPMS_HANKEL_PLUS only; points in the plane
+ * of the lattice, as the e32 arrays take in-plane shifts only). */
+ccart3_t qpms_scatsyswk_scattered_E(const qpms_scatsys_at_omega_k_t *sswk, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where);
 /* Extension: qpms_scatsys_scattered_E at npts points in one launch (host arrays). */
 void qpms_hip_scatsys_scattered_E_points(const qpms_scatsys_t *ss, qpms_bessel_t btyp, _Complex double k,
 		const _Complex double *cvf, size_t npts, const cart3_t *where, ccart3_t *out);

@@ -794,3 +794,66 @@ ccart3_t qpms_scatsysw_scattered_E__alt(const qpms_scatsys_at_omega_t *ssw, qpms
 		const _Complex double *cvf, const cart3_t where) {
 	return qpms_scatsys_scattered_E__alt(ssw->ss, btyp, ssw->wavenumber, cvf, where);
 }
+
+/* scatsystem.c:2113-2188: periodic systems, through the lattice-summed
+ * translation arrays (destination = regular electric dipoles, source = the
+ * particle's basis, shift = point - particle) times the dipoles' value at the
+ * origin. Points outside the lattice plane need out-of-plane Ewald shifts,
+ * which the e32 arrays reject. */
+ccart3_t qpms_scatsyswk_scattered_E(const qpms_scatsys_at_omega_k_t *sswk, qpms_bessel_t btyp,
+		const _Complex double *cvf, const cart3_t where) {
+	QH_ENSURE(btyp == QPMS_HANKEL_PLUS, "Only scattered field with first kind Hankel functions currently implemented.");
+	const qpms_scatsys_t *ss = sswk->ssw->ss;
+	QH_ENSURE(ss->lattice_dimension == 2, "Only 2D-periodic lattices implemented");
+	QH_ENSURE(sswk->k[2] == 0 && ss->per.lattice_basis[0].z == 0 && ss->per.lattice_basis[1].z == 0,
+		"lattice and Bloch vector must lie in the xy plane");
+	QH_ENSURE(ss->c->normalisation == QPMS_NORMALISATION_DEFAULT,
+		"the scattered field is implemented for the default VSWF normalisation only");
+	const complex double k = sswk->ssw->wavenumber;
+	/* the regular electric dipoles N_{1,-1..1} at r = 0 (theta = pi/2, phi = 0) */
+	qpms_uvswfi_t dil[3] = {qpms_tmn2uvswfi(QPMS_VSWF_ELECTRIC, -1, 1), qpms_tmn2uvswfi(QPMS_VSWF_ELECTRIC, 0, 1),
+		qpms_tmn2uvswfi(QPMS_VSWF_ELECTRIC, 1, 1)};
+	qpms_vswf_set_spec_t dipspec;
+	memset(&dipspec, 0, sizeof dipspec);
+	dipspec.n = 3; dipspec.ilist = dil; dipspec.lMax = 1; dipspec.lMax_M = 0; dipspec.lMax_N = 1;
+	dipspec.lMax_L = -1; dipspec.norm = ss->c->normalisation;
+	ccart3_t dip[3];
+	{
+		/* three unit coefficient vectors of lMax 1 (6 complex each), the origin
+		 * (3 doubles, padded to 4 so that the results stay 16-byte aligned) */
+		double h[3 * 12 + 4] = {0};
+		for (int j = 0; j < 3; ++j) h[12 * j + 2 * j] = 1;
+		double *d = qh_dmalloc(sizeof h + 3 * sizeof(ccart3_t));
+		double *dcv = d, *dw = d + 36, *dout = d + 40;
+		qh_h2d(d, h, sizeof h, NULL);
+		for (int j = 0; j < 3; ++j)
+			qpms_hip_scattered_E_dev(1, 1, dw, 0, QPMS_BESSEL_REGULAR, dcv + 12 * j, 1, dw, dout + 6 * j, 0, NULL);
+		qh_d2h(dip, dout, 3 * sizeof(ccart3_t), NULL);
+		qh_sync(NULL);
+		qh_dfree(d);
+	}
+	const double maxR = sqrt(ss->per.unitcell_volume) * 64;   /* scatsystem.c:2160-2161 */
+	const double maxK = 2048 * 2 * M_PI / maxR;
+	const cart2_t b1 = {ss->per.lattice_basis[0].x, ss->per.lattice_basis[0].y};
+	const cart2_t b2 = {ss->per.lattice_basis[1].x, ss->per.lattice_basis[1].y};
+	const cart2_t beta = {sswk->k[0], sswk->k[1]};
+	complex double *s;
+	QH_MALLOC(s, ss->max_bspecn * 3 * sizeof(complex double));
+	complex double e[3] = {0, 0, 0};
+	for (qpms_ss_pi_t pi = 0; pi < ss->p_count; ++pi) {
+		const qpms_vswf_set_spec_t *bspec = qh_bspec_pi(ss, pi);
+		const cart3_t pp = ss->p[pi].pos;
+		const cart3_t sh = {where.x - pp.x, where.y - pp.y, where.z - pp.z};
+		QH_ENSURE(qpms_trans_calculator_get_trans_array_e32(ss->c, s, NULL, &dipspec, 1, bspec, 3, sswk->eta, k,
+			b1, b2, beta, sh, maxR, maxK) == QPMS_SUCCESS, "lattice-summed translation array failed");
+		const complex double *cv = cvf + ss->fecv_pstarts[pi];
+		for (size_t i = 0; i < bspec->n; ++i)
+			for (int j = 0; j < 3; ++j) {
+				const complex double f = cv[i] * s[3 * i + j];
+				e[0] += f * dip[j].x; e[1] += f * dip[j].y; e[2] += f * dip[j].z;
+			}
+	}
+	free(s);
+	ccart3_t r = {e[0], e[1], e[2]};
+	return r;
+}

@@ -4,11 +4,14 @@
  * qpms_scatsyswk_build_modeproblem_matrix_full at a Bloch vector, then the
  * FP64 LU of M and log|det M|.
  *
- * usage: ss_periodic OUT a lMax radius epsin_re epsin_im eps_bg eV beta_x beta_y
+ * usage: ss_periodic OUT a lMax radius epsin_re epsin_im eps_bg eV beta_x beta_y [POINTS]
  * Hexagonal lattice b1 = (a, 0), b2 = (a/2, a sqrt3/2) with particles at
  * (0,0,0) and (a/sqrt3, 0, 0). Writes int64 L, double eta, double
  * wavenumber (re, im), double lattice_basis[2][3], complex128 W[L*L], M[L*L],
- * T diag[nb], LU[L*L], int32 ipiv[L]. */
+ * T diag[nb], LU[L*L], int32 ipiv[L]. With POINTS (float64 x, y, z per
+ * point) it appends int64 npts and qpms_scatsyswk_scattered_E of the
+ * coefficient vector c_i = exp(0.7 i I) / (1 + i) at each point (complex128
+ * x, y, z). */
 #include <complex.h>
 #include <math.h>
 #include <stdio.h>
@@ -19,8 +22,8 @@
 #define EV_RAD_PER_S (1.602176634e-19 / 1.054571817e-34)
 
 int main(int argc, char **argv) {
-	if (argc != 11) {
-		fprintf(stderr, "usage: %s OUT a lMax radius epsin_re epsin_im eps_bg eV beta_x beta_y\n", argv[0]);
+	if (argc != 11 && argc != 12) {
+		fprintf(stderr, "usage: %s OUT a lMax radius epsin_re epsin_im eps_bg eV beta_x beta_y [POINTS]\n", argv[0]);
 		return 2;
 	}
 	const double a = atof(argv[2]);
@@ -79,6 +82,22 @@ int main(int argc, char **argv) {
 		&& fwrite(W, sizeof *W, L * L, fo) == L * L && fwrite(M, sizeof *M, L * L, fo) == L * L
 		&& fwrite(td, sizeof *td, nb, fo) == nb && fwrite(lu.a, sizeof *lu.a, L * L, fo) == L * L
 		&& fwrite(lu.ipiv, sizeof *lu.ipiv, L, fo) == L;
+	if (ok && argc == 12) {
+		FILE *fp = fopen(argv[11], "rb");
+		if (!fp) { perror(argv[11]); return 4; }
+		fseek(fp, 0, SEEK_END);
+		const long long npts = ftell(fp) / (long long)sizeof(cart3_t);
+		fseek(fp, 0, SEEK_SET);
+		cart3_t *w = malloc((size_t)npts * sizeof *w);
+		ccart3_t *e = malloc((size_t)npts * sizeof *e);
+		complex double *cv = malloc(L * sizeof *cv);
+		for (size_t i = 0; i < L; ++i) cv[i] = cexp(0.7 * I * i) / (1.0 + i);
+		ok = fread(w, sizeof *w, (size_t)npts, fp) == (size_t)npts;
+		fclose(fp);
+		for (long long t = 0; ok && t < npts; ++t) e[t] = qpms_scatsyswk_scattered_E(&sswk, QPMS_HANKEL_PLUS, cv, w[t]);
+		ok = ok && fwrite(&npts, sizeof npts, 1, fo) == 1 && fwrite(e, sizeof *e, (size_t)npts, fo) == (size_t)npts;
+		free(w); free(e); free(cv);
+	}
 	fclose(fo);
 	printf("L=%zu eta=%.17g\n", L, sswk.eta);
 	free(td); free(W); free(M);

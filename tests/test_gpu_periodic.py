@@ -24,10 +24,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "c", "_build", "ss_periodic")
 
 
-def run_driver(tmp_path, lMax, radius, eps_in, eps_bg, eV, beta):
+def run_driver(tmp_path, lMax, radius, eps_in, eps_bg, eV, beta, points=None):
     out = tmp_path / "per.bin"
-    subprocess.run([DRIVER, str(out), repr(ec.A_LAT), str(lMax), repr(radius), repr(eps_in.real), repr(eps_in.imag),
-                    repr(eps_bg), repr(eV), repr(beta[0]), repr(beta[1])], check=True, timeout=300)
+    args = [DRIVER, str(out), repr(ec.A_LAT), str(lMax), repr(radius), repr(eps_in.real), repr(eps_in.imag),
+            repr(eps_bg), repr(eV), repr(beta[0]), repr(beta[1])]
+    if points is not None:
+        pf = tmp_path / "points.bin"
+        pf.write_bytes(np.ascontiguousarray(points, np.float64).tobytes())
+        args.append(str(pf))
+    subprocess.run(args, check=True, timeout=300)
     raw = out.read_bytes()
     L = int(np.frombuffer(raw[:8], np.int64)[0])
     hdr = np.frombuffer(raw[8:80], np.float64)
@@ -41,8 +46,13 @@ def run_driver(tmp_path, lMax, radius, eps_in, eps_bg, eV, beta):
     LU = np.frombuffer(raw[o:o + 16 * L * L], np.complex128).reshape(L, L)
     o += 16 * L * L
     ipiv = np.frombuffer(raw[o:o + 4 * L], np.int32)
-    return dict(L=L, eta=hdr[0], k=complex(hdr[1], hdr[2]), basis=hdr[3:].reshape(2, 3), W=W, M=M, tdiag=td,
-                LU=LU, ipiv=ipiv)
+    o += 4 * L
+    r = dict(L=L, eta=hdr[0], k=complex(hdr[1], hdr[2]), basis=hdr[3:].reshape(2, 3), W=W, M=M, tdiag=td,
+             LU=LU, ipiv=ipiv)
+    if points is not None:
+        npts = int(np.frombuffer(raw[o:o + 8], np.int64)[0])
+        r["E"] = np.frombuffer(raw[o + 8:], np.complex128).reshape(npts, 3)
+    return r
 
 
 def sigma_total(LE, k, beta, sh, eta):
@@ -110,3 +120,35 @@ def test_periodic_modeproblem_abi(gpu, tmp_path, lMax, beta):
     assert rel(Lm @ Um, PA) < 1e-12
     ld = np.sum(np.log(np.abs(np.diag(Um))))
     assert abs(ld - np.linalg.slogdet(Mo)[1]) < 1e-9 * max(1, abs(ld))
+
+
+@pytest.mark.parametrize("lMax,beta", [(2, (0.25, 0.1)), (3, (0.0, 0.0))])
+def test_periodic_scattered_field_abi(gpu, tmp_path, lMax, beta):
+    """qpms_scatsyswk_scattered_E at in-plane points against the oracle's
+    lattice-summed translation arrays (destination: the regular electric
+    dipoles at the point; source: each particle's basis; shift: point -
+    particle) contracted with the coefficients and the dipoles' value at the
+    origin (scatsystem.c:2113-2188)."""
+    b = (beta[0] * 2 * np.pi / ec.A_LAT, beta[1] * 2 * np.pi / ec.A_LAT)
+    pts = np.array([[60e-9, 140e-9, 0.0], [-150e-9, -40e-9, 0.0], [ec.A_LAT / np.sqrt(3) + 90e-9, 70e-9, 0.0]])
+    g = run_driver(tmp_path, lMax, 50e-9, -12 + 1j, 1.52 ** 2, 2.0, b, points=pts)
+    k, eta = g["k"], g["eta"]
+    ne = lMax * (lMax + 2)
+    L = g["L"]
+    cv = np.exp(0.7j * np.arange(L)) / (1.0 + np.arange(L))
+    dip = np.array([of.scattered_E(1, [[0, 0, 0]], k, np.eye(6)[j], [[0, 0, 0]], J=1)[0] for j in range(3)])
+    pos = np.array([[0.0, 0.0], [ec.A_LAT / np.sqrt(3), 0.0]])
+    o = of.oracle()
+    o.or_e32_combine.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    oc = of.OracleCalc(lMax)
+    ref = np.zeros((len(pts), 3), np.complex128)
+    for t, w in enumerate(pts):
+        for p in range(2):
+            sig = sigma_total(2 * lMax + 1, k, b, tuple(w[:2] - pos[p]), eta)
+            A = np.empty((ne, ne), np.complex128)
+            B = np.empty((ne, ne), np.complex128)
+            o.or_e32_combine(oc.c, sig.ctypes.data, 0, A.ctypes.data, B.ctypes.data)
+            c = cv[p * 2 * ne:(p + 1) * 2 * ne]
+            for j in range(3):
+                ref[t] += (A[j] @ c[:ne] + B[j] @ c[ne:]) * dip[j]
+    assert rel(g["E"], ref) < 1e-10
