@@ -543,7 +543,7 @@ ccart3_t qpms_scatsys_scattered_E(const qpms_scatsys_t *ss, qpms_bessel_t btyp, 
 /* scatsystem.h:734-740, scatsystem.c:2039-2045 (finite systems only, aborts otherwise) */
 ccart3_t qpms_scatsysw_scattered_E(const qpms_scatsys_at_omega_t *ssw, qpms_bessel_t btyp,
 		const _Complex double *cvf, const cart3_t where);
-/* scatsystem.h:750-756, scatsystem.c:2048-2103: the reference's translation-based
+/* scatsystem.h:750-756, scatsystem.c:2048-2109: the reference's translation-based
  * variant. It translates with the vector particle - point, which gives the direct
  * sum with electric coefficients times (-1)^(l+1) and magnetic ones times (-1)^l;
  * that value is returned (QPMS_HIP_SCATTERED_E_ALT=direct returns the direct sum). */
