@@ -143,6 +143,28 @@ QPMS_HD inline cplx gamma_inc_half(int j, cplx z, int m) {
   return val;
 }
 
+// Gamma(1/2 - j, x) for real x > 0.5 on the principal branch: the continued
+// fraction of gamma_inc_half in real arithmetic (every quantity is real
+// there), for the evanescent orders of a real wavenumber.
+QPMS_HD inline double gamma_inc_half_real(int j, double x) {
+  const double a = 0.5 - j;
+  const double tiny = 1e-300;
+  double hn = 1.0, Cn = 1.0 / tiny, Dn = 1.0;
+  const double xi = 1.0 / x;
+  for (int n = 2; n < 5000; ++n) {
+    const double an = (n % 2) ? (0.5 * (n - 1)) * xi : (0.5 * n - a) * xi;
+    Dn = 1.0 + an * Dn;
+    if (fabs(Dn) < tiny) Dn = tiny;
+    Cn = 1.0 + an / Cn;
+    if (fabs(Cn) < tiny) Cn = tiny;
+    Dn = 1.0 / Dn;
+    const double delta = Cn * Dn;
+    hn *= delta;
+    if (fabs(delta - 1.0) < 2.2204460492503131e-16) break;
+  }
+  return hn * exp((a - 1.0) * log(x) - x);
+}
+
 // clilgamma(z) (ewald.h:157-170): sqrt(z-1) sqrt(z+1) with the branches
 // -pi/2 < arg(z+1) < 3pi/2 and -3pi/2 < arg(z-1) < pi/2.
 QPMS_HD inline cplx clilgamma(cplx z) {

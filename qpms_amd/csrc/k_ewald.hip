@@ -166,8 +166,13 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   }
   if (z.re > 740.0) return false;  // every Gamma(1/2-j, z) underflows
   cplx G[JM + 1];
+  if (branch == 0 && z.im == 0.0 && z.re > 0.5) {
 #pragma unroll
-  for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
+    for (int j = 0; j <= JM; ++j) G[j] = mk(qew::gamma_inc_half_real(j, z.re), 0.0);
+  } else {
+#pragma unroll
+    for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
+  }
   // gamma^{2j-1}
   cplx gp[JM + 1];
   gp[0] = cinv(gam);
