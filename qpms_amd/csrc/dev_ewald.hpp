@@ -145,8 +145,9 @@ QPMS_HD inline cplx gamma_inc_half(int j, cplx z, int m) {
 
 // Gamma(1/2 - j, x) for real x > 0.5 on the principal branch: the continued
 // fraction of gamma_inc_half in real arithmetic (every quantity is real
-// there), for the evanescent orders of a real wavenumber.
-QPMS_HD inline double gamma_inc_half_real(int j, double x) {
+// there), for the evanescent orders of a real wavenumber. Returns the
+// fraction F; Gamma = x^{-1/2-j} e^{-x} F.
+QPMS_HD inline double gamma_inc_half_real_cf(int j, double x) {
   const double a = 0.5 - j;
   const double tiny = 1e-300;
   double hn = 1.0, Cn = 1.0 / tiny, Dn = 1.0;
@@ -162,7 +163,11 @@ QPMS_HD inline double gamma_inc_half_real(int j, double x) {
     hn *= delta;
     if (fabs(delta - 1.0) < 2.2204460492503131e-16) break;
   }
-  return hn * exp((a - 1.0) * log(x) - x);
+  return hn;
+}
+
+QPMS_HD inline double gamma_inc_half_real(int j, double x) {
+  return gamma_inc_half_real_cf(j, x) * exp((-0.5 - j) * log(x) - x);
 }
 
 // clilgamma(z) (ewald.h:157-170): sqrt(z-1) sqrt(z+1) with the branches

@@ -201,6 +201,30 @@ QPMS_HD inline void sph_jy_cplx(cplx z, cplx *j, cplx *y) {
   }
 }
 
+// Spherical Hankel functions h^(1) (kind 3) or h^(2) (kind 4), l = 0..L, at
+// nonzero x directly (DLMF 10.49.6, 10.49.1): h0 = -i e^{iz}/z,
+// h1 = -e^{iz}(z + i)/z^2 (and the conjugate forms for h^(2)), then upward
+// recurrence, which is stable for h. Forming j + i y would lose everything to
+// cancellation once |Im z| is large; at real z the result agrees with
+// j + i y to rounding relative to |h|.
+template <int L>
+QPMS_HD inline void sph_hankel_direct(int kind, cplx x, cplx *out) {
+  const double sg = (kind == 3) ? 1.0 : -1.0;
+  const cplx e = cexp_(mk(-sg * x.im, sg * x.re));   // e^{+-iz}
+  const cplx zi = cinv(x);
+  cplx h0 = mk(sg * (e * zi).im, -sg * (e * zi).re);  // -+i e^{+-iz}/z
+  cplx h1 = -(e * (x + mk(0.0, sg)) * (zi * zi));
+  out[0] = h0;
+  if (L >= 1) out[1] = h1;
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    const cplx h2 = (double)(2 * l + 1) * (zi * h1) - h0;
+    out[l + 1] = h2;
+    h0 = h1;
+    h1 = h2;
+  }
+}
+
 // z_l for the requested kind (1 = j, 2 = y, 3 = h+, 4 = h-), l = 0..L.
 // Real argument when im == 0 exactly (the reference branches the same way,
 // bessel.c:87-88).
@@ -223,24 +247,7 @@ QPMS_HD inline void sph_bessel_fill(int kind, cplx x, cplx *out) {
       out[l] = mk(re, im);
     }
   } else if (kind >= 3) {
-    // Hankel functions of complex argument directly (DLMF 10.49.6, 10.49.1):
-    // h0 = -i e^{iz}/z, h1 = -e^{iz}(z + i)/z^2 (and the conjugate forms for
-    // h^(2)), then upward recurrence, which is stable for h. Forming j + i y
-    // would lose everything to cancellation once |Im z| is large.
-    const double sg = (kind == 3) ? 1.0 : -1.0;
-    const cplx e = cexp_(mk(-sg * x.im, sg * x.re));   // e^{+-iz}
-    const cplx zi = cinv(x);
-    cplx h0 = mk(sg * (e * zi).im, -sg * (e * zi).re);  // -+i e^{+-iz}/z
-    cplx h1 = -(e * (x + mk(0.0, sg)) * (zi * zi));
-    out[0] = h0;
-    if (L >= 1) out[1] = h1;
-#pragma unroll
-    for (int l = 1; l < L; ++l) {
-      const cplx h2 = (double)(2 * l + 1) * (zi * h1) - h0;
-      out[l + 1] = h2;
-      h0 = h1;
-      h1 = h2;
-    }
+    sph_hankel_direct<L>(kind, x, out);
   } else {
     cplx j[L + 1], y[L + 1];
     sph_jy_cplx<L>(x, j, y);

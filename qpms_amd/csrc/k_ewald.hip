@@ -167,8 +167,14 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   if (z.re > 740.0) return false;  // every Gamma(1/2-j, z) underflows
   cplx G[JM + 1];
   if (branch == 0 && z.im == 0.0 && z.re > 0.5) {
+    // x^{-1/2-j} e^{-x} from one exp and one square root, then x^{-1} per j
+    const double xi = 1.0 / z.re;
+    double pre = exp(-z.re) * sqrt(xi);
 #pragma unroll
-    for (int j = 0; j <= JM; ++j) G[j] = mk(qew::gamma_inc_half_real(j, z.re), 0.0);
+    for (int j = 0; j <= JM; ++j) {
+      G[j] = mk(qew::gamma_inc_half_real_cf(j, z.re) * pre, 0.0);
+      pre *= xi;
+    }
   } else {
 #pragma unroll
     for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);

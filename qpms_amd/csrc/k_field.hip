@@ -144,7 +144,9 @@ __global__ void k_neg_re_cdot(long n, const v2d *__restrict__ a, long sa, const 
 // One particle's field at offset (dx, dy, dz) from it, added to E (cartesian).
 // c: the particle's 2 NE coefficients (electric block, then magnetic);
 // parity: multiply electric coefficients by (-1)^(l+1), magnetic by (-1)^l.
-template <int LMAX>
+// HANKEL: kind is 3 or 4 and h is formed directly at every argument (the
+// outgoing field's case; the origin is then a singular point, NaN).
+template <int LMAX, bool HANKEL>
 __device__ __forceinline__ void field_of_particle(double dx, double dy, double dz, cplx k, int kind,
                                                   const v2d *__restrict__ c, bool parity, cplx *E) {
   constexpr int NE = LMAX * (LMAX + 2);
@@ -155,8 +157,11 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
   const double ct = cos(theta), st = sin(theta), cf = cos(phi), sf = sin(phi);
   const cplx kr = k * r;
   cplx z[LMAX + 1];
-  qsf::sph_bessel_fill<LMAX>(kind, kr, z);
   const bool origin = (kr.re == 0.0 && kr.im == 0.0);
+  if (HANKEL && !origin)
+    qsf::sph_hankel_direct<LMAX>(kind, kr, z);
+  else
+    qsf::sph_bessel_fill<LMAX>(kind, kr, z);
   const cplx krinv = origin ? mk(0.0, 0.0) : cinv(kr);
   const bool pole = (fabs(ct) == 1.0);   // legendre.c:65-80
   double P[qsf::leg_n(LMAX)];
@@ -217,7 +222,7 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
   E[2] = E[2] + ct * Er - st * Et;
 }
 
-template <int LMAX, bool WAVE>
+template <int LMAX, bool WAVE, bool HANKEL>
 __global__ __launch_bounds__(256) void k_scattered_e(int npts, const double *__restrict__ where, int npart,
                                                      const double *__restrict__ pos, double k_re, double k_im,
                                                      int kind, const v2d *__restrict__ cvf, int parity,
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(256) void k_scattered_e(int npts, const double *__r
   const cplx k = mk(k_re, k_im);
   cplx E[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
   for (int p = WAVE ? lane : 0; p < npart; p += WAVE ? 64 : 1)
-    field_of_particle<LMAX>(wx - pos[3 * p], wy - pos[3 * p + 1], wz - pos[3 * p + 2], k, kind,
+    field_of_particle<LMAX, HANKEL>(wx - pos[3 * p], wy - pos[3 * p + 1], wz - pos[3 * p + 2], k, kind,
                             cvf + (size_t)p * NB, parity != 0, E);
   if (WAVE) {
 #pragma unroll
@@ -303,17 +308,22 @@ extern "C" int qh_launch_scattered_e(int lMax, size_t npts, const double *where_
   hipStream_t s = (hipStream_t)qh_stream(stream);
   // a wave per point when points are too few to fill the device thread-per-point
   const bool wave = npart >= 32 && npts < 262144;
+  const bool hankel = kind >= 3;
   const dim3 grid((unsigned)(wave ? (npts + 3) / 4 : (npts + 255) / 256)), block(256);
   const v2d *c = (const v2d *)cvf_dev;
   v2d *o = (v2d *)out_dev;
-#define QH_SCE(LM)                                                                                              \
-  do {                                                                                                          \
-    if (wave)                                                                                                   \
-      hipLaunchKernelGGL((k_scattered_e<LM, true>), grid, block, 0, s, (int)npts, where_dev, (int)npart, pos_dev, \
-                         k_re, k_im, kind, c, parity, o);                                                       \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_scattered_e<LM, false>), grid, block, 0, s, (int)npts, where_dev, (int)npart,       \
-                         pos_dev, k_re, k_im, kind, c, parity, o);                                              \
+#define QH_SCE1(LM, W, H) \
+  hipLaunchKernelGGL((k_scattered_e<LM, W, H>), grid, block, 0, s, (int)npts, where_dev, (int)npart, pos_dev, k_re, \
+                     k_im, kind, c, parity, o)
+#define QH_SCE(LM)                                   \
+  do {                                               \
+    if (wave) {                                      \
+      if (hankel) QH_SCE1(LM, true, true);           \
+      else QH_SCE1(LM, true, false);                 \
+    } else {                                         \
+      if (hankel) QH_SCE1(LM, false, true);          \
+      else QH_SCE1(LM, false, false);                \
+    }                                                \
   } while (0)
   switch (lMax) {
     case 1: QH_SCE(1); break;
@@ -323,6 +333,7 @@ extern "C" int qh_launch_scattered_e(int lMax, size_t npts, const double *where_
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", lMax);
   }
 #undef QH_SCE
+#undef QH_SCE1
   HIPCHK(hipGetLastError());
   return 0;
 }
