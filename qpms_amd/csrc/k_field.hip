@@ -151,10 +151,20 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
                                                   const v2d *__restrict__ c, bool parity, cplx *E) {
   constexpr int NE = LMAX * (LMAX + 2);
   constexpr double PI_ = 3.14159265358979323846;
+  // direction cosines of cart2sph's (theta, phi) (vectors.h:134-140): theta =
+  // pi/2 at r = 0, phi = atan2(dy, dx) (also on the z axis)
   const double r = sqrt(dx * dx + dy * dy + dz * dz);
-  const double theta = r != 0.0 ? acos(dz / r) : PI_ / 2;   // cart2sph (vectors.h:134-140)
-  const double phi = atan2(dy, dx);
-  const double ct = cos(theta), st = sin(theta), cf = cos(phi), sf = sin(phi);
+  const double rho = sqrt(dx * dx + dy * dy);
+  const double ct = r != 0.0 ? dz / r : cos(PI_ / 2), st = r != 0.0 ? rho / r : 1.0;
+  double cf, sf;
+  if (rho != 0.0) {
+    cf = dx / rho;
+    sf = dy / rho;
+  } else {
+    const double phi = atan2(dy, dx);
+    cf = cos(phi);
+    sf = sin(phi);
+  }
   const cplx kr = k * r;
   cplx z[LMAX + 1];
   const bool origin = (kr.re == 0.0 && kr.im == 0.0);
@@ -174,6 +184,10 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
     const cplx besder = z[l - 1] - (double)l * besfac;
     const double pe = (parity && !(l & 1)) ? -1.0 : 1.0;   // electric: (-1)^(l+1)
     const double pm = (parity && (l & 1)) ? -1.0 : 1.0;    // magnetic: (-1)^l
+    // angular sums of degree l; the radial factors multiply them once per l:
+    // N = (l(l+1) leg z/kr e, tau besder e, pi besder i e), M = (0, pi z i e, -tau z e)
+    cplx sLegN = mk(0.0, 0.0), sTauN = mk(0.0, 0.0), sPiN = mk(0.0, 0.0), sPiM = mk(0.0, 0.0),
+         sTauM = mk(0.0, 0.0);
     cplx em = mk(1.0, 0.0);   // e^{i m phi}
 #pragma unroll
     for (int m = 0; m <= l; ++m) {
@@ -207,15 +221,18 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
         const cplx e = sgn ? conj(em) : em;
         const int y = l * (l + 1) + mm - 1;
         const v2d cn = c[y], cm = c[NE + y];
-        const cplx cN = mk(pe * cn.x, pe * cn.y), cM = mk(pm * cm.x, pm * cm.y);
-        const cplx be = besfac * e, bde = besder * e, ze = z[l] * e;
-        const cplx nbde = cN * bde, mze = cM * ze;
-        Er = Er + ((double)(l * (l + 1)) * lg) * (cN * be);
-        // N: (l(l+1) leg z/kr e, tau besder e, pi besder i e); M: (0, pi z i e, -tau z e)
-        Et = Et + ta * nbde + p_ * mk(-mze.im, mze.re);
-        Ef = Ef + p_ * mk(-nbde.im, nbde.re) - ta * mze;
+        const cplx u = mk(pe * cn.x, pe * cn.y) * e, v = mk(pm * cm.x, pm * cm.y) * e;
+        sLegN = sLegN + lg * u;
+        sTauN = sTauN + ta * u;
+        sPiN = sPiN + p_ * u;
+        sPiM = sPiM + p_ * v;
+        sTauM = sTauM + ta * v;
       }
     }
+    const cplx zs = z[l] * sPiM, bs = besder * sPiN;
+    Er = Er + (double)(l * (l + 1)) * (besfac * sLegN);
+    Et = Et + besder * sTauN + mk(-zs.im, zs.re);
+    Ef = Ef + mk(-bs.im, bs.re) - z[l] * sTauM;
   }
   E[0] = E[0] + (st * cf) * Er + (ct * cf) * Et - sf * Ef;
   E[1] = E[1] + (st * sf) * Er + (ct * sf) * Et + cf * Ef;
