@@ -168,10 +168,16 @@ __device__ __forceinline__ void field_of_particle(double dx, double dy, double d
   const cplx kr = k * r;
   cplx z[LMAX + 1];
   const bool origin = (kr.re == 0.0 && kr.im == 0.0);
-  if (HANKEL && !origin)
-    qsf::sph_hankel_direct<LMAX>(kind, kr, z);
-  else
+  if (HANKEL) {
+    if (origin) {   // h_l is singular at the origin (as sph_bessel_fill gives it)
+#pragma unroll
+      for (int l = 0; l <= LMAX; ++l) z[l] = mk(NAN, NAN);
+    } else {
+      qsf::sph_hankel_direct<LMAX>(kind, kr, z);
+    }
+  } else {
     qsf::sph_bessel_fill<LMAX>(kind, kr, z);
+  }
   const cplx krinv = origin ? mk(0.0, 0.0) : cinv(kr);
   const bool pole = (fabs(ct) == 1.0);   // legendre.c:65-80
   double P[qsf::leg_n(LMAX)];
