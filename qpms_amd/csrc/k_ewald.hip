@@ -167,13 +167,21 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   if (z.re > 740.0) return false;  // every Gamma(1/2-j, z) underflows
   cplx G[JM + 1];
   if (branch == 0 && z.im == 0.0 && z.re > 0.5) {
-    // x^{-1/2-j} e^{-x} from one exp and one square root, then x^{-1} per j
+    // pre_j = x^{-1/2-j} e^{-x} from one exp and one square root; the
+    // continued fraction for the lowest order only, then upward in the order,
+    // Gamma(a+1, x) = a Gamma(a, x) + x^a e^{-x} (a = 1/2 - j; for x > 1/2 the
+    // added term dominates, so no digits are lost)
     const double xi = 1.0 / z.re;
-    double pre = exp(-z.re) * sqrt(xi);
+    double pre[JM + 1];
+    pre[0] = exp(-z.re) * sqrt(xi);
 #pragma unroll
-    for (int j = 0; j <= JM; ++j) {
-      G[j] = mk(qew::gamma_inc_half_real_cf(j, z.re) * pre, 0.0);
-      pre *= xi;
+    for (int j = 1; j <= JM; ++j) pre[j] = pre[j - 1] * xi;
+    double g = qew::gamma_inc_half_real_cf(JM, z.re) * pre[JM];
+    G[JM] = mk(g, 0.0);
+#pragma unroll
+    for (int j = JM; j >= 1; --j) {
+      g = (0.5 - j) * g + pre[j - 1];
+      G[j - 1] = mk(g, 0.0);
     }
   } else {
 #pragma unroll
