@@ -80,7 +80,9 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
     cplx In[LE + 1];
     qew::sr_integrals<LE>(r, k, eta, c_weid, In);
     const cplx eb = cexpi(T[0] * rx + T[1] * ry + T[2] * rz);
-    const cplx e1 = cexpi(atan2(ry, rx));
+    // e^{i phi}, phi = atan2(ry, rx); on the z axis only m = 0 terms survive
+    const double rho = sqrt(rx * rx + ry * ry);
+    const cplx e1 = (rho != 0.0) ? mk(rx / rho, ry / rho) : mk(1.0, 0.0);
     double P[XY ? 1 : qsf::leg_n(LE)];
     if (!XY) {
       const double x = rz / r;
@@ -142,7 +144,6 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   const bool kreal = (T[4] == 0.0);
   const double bx = Kx + T[0], by = Ky + T[1];
   const double rb = sqrt(bx * bx + by * by);
-  const double alpha = atan2(by, bx);
   // gamma, x^2 and the incomplete-gamma branch (ewald.c:384-396)
   cplx gam, z;
   int branch;
@@ -199,7 +200,8 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   rkp[0] = mk(1.0, 0.0);
 #pragma unroll
   for (int p = 1; p <= LE; ++p) rkp[p] = rkp[p - 1] * rk;
-  const cplx e1 = cexpi(alpha);
+  // e^{i alpha}, alpha = atan2(by, bx) (0 where beta + K vanishes)
+  const cplx e1 = (rb != 0.0) ? mk(bx / rb, by / rb) : mk(1.0, 0.0);
   cplx em[LE + 1];
   em[0] = mk(1.0, 0.0);
 #pragma unroll

@@ -742,6 +742,8 @@ int qpms_hip_scattered_E_dev(int lMax, size_t npart, const double *pos_dev, _Com
 		const double *cvf_dev, size_t npts, const double *where_dev, double *out_dev, int parity, void *stream) {
 	QH_ENSURE(lMax >= 1 && lMax <= 4, "lMax %d outside 1..4", lMax);
 	QH_ENSURE(btyp >= QPMS_BESSEL_REGULAR && btyp <= QPMS_HANKEL_MINUS, "invalid Bessel type %d", (int)btyp);
+	QH_ENSURE(npts == 0 || (where_dev && out_dev && (npart == 0 || (pos_dev && cvf_dev))),
+		"scattered field: NULL device buffer for %zu points, %zu particles", npts, npart);
 	return qh_launch_scattered_e(lMax, npts, where_dev, npart, pos_dev, creal(k), cimag(k), (int)btyp, cvf_dev,
 		parity, out_dev, stream);
 }
