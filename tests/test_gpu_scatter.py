@@ -19,6 +19,7 @@ import oracle_ffi as of
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "c", "_build", "ss_finite")
+MIXED = os.path.join(ROOT, "tests", "c", "_build", "ss_field_mixed")
 
 
 class ccart3(ctypes.Structure):
@@ -155,3 +156,34 @@ def test_scattered_field_abi(gpu, tmp_path, shape, lMax, kd, E):
         for t in (-2, -1):
             assert np.max(np.abs(g[key][t] - ref[t])) < 1e-10 * np.max(np.abs(ref[t])), (key, t)
     assert rel(g["E_alt"][near], alt[near]) < 1e-10
+
+
+def test_scattered_field_mixed_bases(gpu, tmp_path):
+    """Particles with bases of lMax 2 and 3 in one system: the library pads
+    each particle's coefficients into the calculator's lMax; compared with the
+    oracle's field of the same padded coefficients."""
+    pos = np.array([[-400e-9, 0, 0], [0, 350e-9, 40e-9], [300e-9, -200e-9, 0]])
+    lms = [2, 3, 2]
+    pts = np.concatenate([_field_points(pos, np.random.default_rng(5), n=20), pos[1:2] + [[0, 0, 90e-9]]])
+    pf, out = tmp_path / "pts.bin", tmp_path / "mixed.bin"
+    pf.write_bytes(np.ascontiguousarray(pts, np.float64).tobytes())
+    subprocess.run([MIXED, str(out), str(pf)], check=True, timeout=120)
+    raw = out.read_bytes()
+    L = int(np.frombuffer(raw[:8], np.int64)[0])
+    k = complex(*np.frombuffer(raw[8:24], np.float64))
+    e = np.frombuffer(raw[24:], np.complex128).reshape(2, len(pts), 3)
+    assert L == sum(2 * l * (l + 2) for l in lms)
+    cv = np.exp(0.37j * np.arange(L)) / (1.0 + 0.1 * np.arange(L))
+    pad = np.zeros((3, 30), np.complex128)
+    off = 0
+    for p, l in enumerate(lms):
+        ne = l * (l + 2)
+        pad[p, :ne] = cv[off:off + ne]
+        pad[p, 15:15 + ne] = cv[off + ne:off + 2 * ne]
+        off += 2 * ne
+    ref = of.scattered_E(3, pos, k, pad.ravel(), pts)
+    near = slice(0, len(pts) - 3)
+    assert rel(e[0][near], ref[near]) < 1e-10
+    assert rel(e[1][near], ref[near]) < 1e-10
+    for t in range(len(pts) - 3, len(pts)):
+        assert np.max(np.abs(e[0][t] - ref[t])) < 1e-10 * np.max(np.abs(ref[t]))
