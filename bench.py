@@ -121,7 +121,8 @@ def measure_w_build(L, ev, stream, nfreq=8):
 
 def measure_peaks(L, ev, stream):
     """Measured counterparts of the datasheet peaks: STREAM triad over three
-    2 GiB arrays (hand-written kernel), FP64 FMA chains (hand-written kernel),
+    2 GiB arrays and a write-only fill of one (hand-written kernels), FP64 FMA
+    chains (hand-written kernel),
     and rocBLAS cgemm / zgemm at order 8192 (the library GEMM the LU's
     trailing update calls)."""
     import qpms_amd
@@ -144,6 +145,9 @@ def measure_peaks(L, ev, stream):
     bufs = [qpms_amd.DeviceBuffer(8 * n) for _ in range(3)]
     ms = timed(lambda: L.qpms_hip_probe_triad(n, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, 0.5, None), 5)
     out["hbm_triad_GBps"] = 24.0 * n / (ms * 1e-3) / 1e9
+    L.qpms_hip_probe_fill.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    ms = timed(lambda: L.qpms_hip_probe_fill(n, bufs[0].ptr, 0.5, None), 5)
+    out["hbm_write_GBps"] = 8.0 * n / (ms * 1e-3) / 1e9
     for b in bufs:
         b.free()
     blocks, iters = 256 * 8, 1 << 15

@@ -709,6 +709,8 @@ beyn_result_t *beyn_solve_v(size_t m, size_t l, beyn_function_M_t M, beyn_functi
 /* ------------------------------------------------------------------------ */
 /* STREAM triad a = b + s c over n doubles (n even; 24 n bytes), device pointers. */
 int qpms_hip_probe_triad(size_t n, double *a, const double *b, const double *c, double s, void *stream);
+/* Write-only stream a = s over n doubles (n even; 8 n bytes), device pointer. */
+int qpms_hip_probe_fill(size_t n, double *a, double s, void *stream);
 /* blocks x 256 lanes of 8 independent FP64 FMA chains for iters steps
  * (16 iters flops per lane); out_dev holds blocks * 256 doubles. */
 int qpms_hip_probe_fma64(int blocks, int iters, double *out_dev, void *stream);

@@ -1,9 +1,12 @@
 // Measured peaks that the roofline fractions are read against, next to the
 // datasheet values (8 TB/s HBM, 78.6 TFLOP/s FP64 vector):
 //
-//  k_triad  STREAM triad a = b + s c over doubles: 16-byte loads and stores,
-//           grid-stride with four loads per array in flight, 24 algorithmic
-//           bytes per element.
+//  k_triad  STREAM triad a = b + s c over doubles: 16-byte non-temporal
+//           loads and stores, one contiguous chunk per workgroup, 24
+//           algorithmic bytes per element.
+//  k_fill   write-only stream a = s over doubles, same layout: the ceiling
+//           of a kernel that only writes, such as the translation-matrix
+//           build.
 //  k_fma64  eight independent FP64 FMA chains per lane; the chains stay
 //           bounded (x -> 0.999 x + 1e-3) and their sum is stored, so no
 //           work is dead. 16 flops per lane per iteration.
@@ -14,25 +17,22 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 
 namespace {
 
+// Each workgroup streams one contiguous chunk of PROBE_CHUNK 16-byte elements
+// (256 KiB per array), its lanes 16 bytes apart.
+constexpr long PROBE_CHUNK = 16384;
 __global__ __launch_bounds__(256) void k_triad(long n2, v2d *__restrict__ a, const v2d *__restrict__ b,
                                                const v2d *__restrict__ c, double s) {
-  // four independent 16-byte loads per array in flight per lane
-  const long stride = (long)gridDim.x * blockDim.x;
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n2; i += 4 * stride) {
-    v2d x[4], y[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      x[u] = __builtin_nontemporal_load(b + i + u * stride);
-      y[u] = __builtin_nontemporal_load(c + i + u * stride);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(x[u] + s * y[u], a + i + u * stride);
-  }
-  for (; i < n2; i += stride) {
+  const long b0 = (long)blockIdx.x * PROBE_CHUNK, b1 = min(b0 + PROBE_CHUNK, n2);
+  for (long i = b0 + threadIdx.x; i < b1; i += 256) {
     const v2d x = __builtin_nontemporal_load(b + i), y = __builtin_nontemporal_load(c + i);
     __builtin_nontemporal_store(x + s * y, a + i);
   }
+}
+
+__global__ __launch_bounds__(256) void k_fill(long n2, v2d *__restrict__ a, double s) {
+  const long b0 = (long)blockIdx.x * PROBE_CHUNK, b1 = min(b0 + PROBE_CHUNK, n2);
+  const v2d v = v2d{s, s};
+  for (long i = b0 + threadIdx.x; i < b1; i += 256) __builtin_nontemporal_store(v, a + i);
 }
 
 __global__ __launch_bounds__(256) void k_fma64(int iters, double *__restrict__ out) {
@@ -60,8 +60,20 @@ extern "C" int qpms_hip_probe_triad(size_t n, double *a, const double *b, const 
   if (n == 0) return 0;
   if (n & 1) qpms_hip_die(__FILE__, __LINE__, __func__, "triad length %zu must be even", n);
   hipStream_t st = (hipStream_t)qh_stream(stream);
-  hipLaunchKernelGGL(k_triad, dim3(256 * 16), dim3(256), 0, st, (long)(n / 2), (v2d *)a, (const v2d *)b,
-                     (const v2d *)c, s);
+  const long n2 = (long)(n / 2);
+  hipLaunchKernelGGL(k_triad, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a,
+                     (const v2d *)b, (const v2d *)c, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// n doubles (even), 8 n bytes written.
+extern "C" int qpms_hip_probe_fill(size_t n, double *a, double s, void *stream) {
+  if (n == 0) return 0;
+  if (n & 1) qpms_hip_die(__FILE__, __LINE__, __func__, "fill length %zu must be even", n);
+  hipStream_t st = (hipStream_t)qh_stream(stream);
+  const long n2 = (long)(n / 2);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
