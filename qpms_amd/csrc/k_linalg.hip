@@ -422,14 +422,14 @@ __device__ __forceinline__ unsigned long long wave_amax(unsigned long long v) {
 // rows); returns their number.
 // Y[c] -= l P[c] for c in [c1, w): one complex value per step, or (FP32)
 // pairs of them as 16-byte LDS accesses.
-__device__ __forceinline__ void ts_axpy(v2f *Y, const v2f *P, v2f l, int c1, int w) {
+__device__ __forceinline__ void ts_axpy(v2f *__restrict__ Y, const v2f *__restrict__ P, v2f l, int c1, int w) {
   int c = c1;
   if ((c & 1) && c < w) {
     const v2f u = P[c], y = Y[c];
     Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
     ++c;
   }
-#pragma unroll 2
+#pragma unroll 4
   for (; c + 1 < w; c += 2) {
     const v4f u = *reinterpret_cast<const v4f *>(P + c);
     v4f y = *reinterpret_cast<v4f *>(Y + c);
@@ -444,7 +444,7 @@ __device__ __forceinline__ void ts_axpy(v2f *Y, const v2f *P, v2f l, int c1, int
     Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
   }
 }
-__device__ __forceinline__ void ts_axpy(v2d *Y, const v2d *P, v2d l, int c1, int w) {
+__device__ __forceinline__ void ts_axpy(v2d *__restrict__ Y, const v2d *__restrict__ P, v2d l, int c1, int w) {
 #pragma unroll 4
   for (int c = c1; c < w; ++c) {
     const v2d u = P[c], y = Y[c];
