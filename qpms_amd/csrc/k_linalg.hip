@@ -257,6 +257,48 @@ __global__ void k_perm_from_ipiv(int n, const int *ipiv, int *perm) {
   }
 }
 
+// C[:, 0:NR] -= A B for few right-hand sides (NR <= 4), FP64 complex: one
+// wave per row of A with its lanes 16 bytes apart along the row (each wave
+// load is 1 KiB contiguous), the NR sums reduced across the wave. A is read
+// once, so the residuals of the iterative refinement run at HBM speed; the
+// tiled GEMM would give each 64-row tile one workgroup and leave most CUs idle.
+template <int NR>
+__global__ __launch_bounds__(256) void k_gemv_sub_f64(int M, int K, const v2d *__restrict__ A, long lda, long sA,
+                                                      const v2d *__restrict__ B, long ldb, long sB,
+                                                      v2d *__restrict__ C, long ldc, long sC) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int z = blockIdx.y;
+  if (i >= M) return;
+  const v2d *a = A + (long)z * sA + (long)i * lda;
+  const v2d *b = B + (long)z * sB;
+  double sr[NR], si[NR];
+#pragma unroll
+  for (int n = 0; n < NR; ++n) sr[n] = si[n] = 0.0;
+#pragma unroll 4
+  for (int k = lane; k < K; k += 64) {
+    const v2d x = __builtin_nontemporal_load(a + k);
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      const v2d y = b[(long)k * ldb + n];
+      sr[n] += x.x * y.x - x.y * y.y;
+      si[n] += x.x * y.y + x.y * y.x;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NR; ++n)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sr[n] += __shfl_xor(sr[n], o);
+      si[n] += __shfl_xor(si[n], o);
+    }
+  if (lane == 0) {
+    v2d *c = C + (long)z * sC + (long)i * ldc;
+#pragma unroll
+    for (int n = 0; n < NR; ++n) c[n] = v2d{c[n].x - sr[n], c[n].y - si[n]};
+  }
+}
+
 __global__ void k_d2f(long n, const v2d *x, v2f *y) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i < n) y[i] = v2f{(float)x[i].x, (float)x[i].y};
@@ -1163,6 +1205,17 @@ extern "C" int qh_getrs_batched_f32(int n, int nrhs, int batch, const float *LU,
 // C -= A B (FP64 complex, batched, row-major, packed leading dimensions)
 extern "C" int qh_gemm_sub_f64(int M, int N, int K, int batch, const double *A, long lda, long sA, const double *B,
                                long ldb, long sB, double *C, long ldc, long sC, void *stream) {
+  if (M <= 0 || N <= 0 || batch <= 0) return 0;
+  if (N <= 4) {   // few right-hand sides: stream A once, a wave per row
+    const dim3 g((unsigned)((M + 3) / 4), (unsigned)batch);
+    hipStream_t s = (hipStream_t)qh_stream(stream);
+#define QH_GV(NR) hipLaunchKernelGGL(k_gemv_sub_f64<NR>, g, dim3(256), 0, s, M, K, (const v2d *)A, lda, sA, \
+                                     (const v2d *)B, ldb, sB, (v2d *)C, ldc, sC)
+    switch (N) { case 1: QH_GV(1); break; case 2: QH_GV(2); break; case 3: QH_GV(3); break; default: QH_GV(4); }
+#undef QH_GV
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_gemm_sub<double>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), M, N, K, (const v2d *)A, lda, sA, (const v2d *)B, ldb, sB,
                      (v2d *)C, ldc, sC);

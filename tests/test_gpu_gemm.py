@@ -29,3 +29,36 @@ def test_gemm_sub(gpu, prec, M, N, K):
     out = dC.to_array(dt, (M, N))
     tol = 1e-13 if prec == "f64" else 2e-5
     assert np.max(np.abs(out - ref)) / max(1.0, np.max(np.abs(ref))) < tol * max(1, K)
+
+
+@pytest.mark.parametrize("N", [1, 2, 4])
+def test_gemm_sub_few_rhs_batched(gpu, N):
+    """FP64 C -= A B with N <= 4 right-hand sides (the refinement residuals:
+    one wave per row of A), batched with per-matrix strides, rows longer and
+    shorter than a wave."""
+    import qpms_amd
+    L = qpms_amd.lib()
+    rng = np.random.default_rng(40 + N)
+    batch, M, K = 3, 777, 1029
+    A = rng.normal(size=(batch, M, K)) + 1j * rng.normal(size=(batch, M, K))
+    B = rng.normal(size=(batch, K, N)) + 1j * rng.normal(size=(batch, K, N))
+    C = rng.normal(size=(batch, M, N)) + 1j * rng.normal(size=(batch, M, N))
+    ref = C - A @ B
+    dA, dB, dC = (qpms_amd.DeviceBuffer.from_array(x) for x in (A, B, C))
+    f = L.qh_gemm_sub_f64
+    lg, vp, i32 = ctypes.c_long, ctypes.c_void_p, ctypes.c_int
+    f.argtypes = [i32, i32, i32, i32, vp, lg, lg, vp, lg, lg, vp, lg, lg, vp]
+    f(M, N, K, batch, dA.ptr, K, M * K, dB.ptr, N, K * N, dC.ptr, N, M * N, None)
+    L.qpms_hip_sync()
+    out = dC.to_array(np.complex128, (batch, M, N))
+    assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-13
+    # a row shorter than a wave
+    A2 = A[0, :5, :7].copy()
+    B2 = B[0, :7].copy()
+    C2 = C[0, :5].copy()
+    d2 = [qpms_amd.DeviceBuffer.from_array(x) for x in (A2, B2, C2)]
+    f(5, N, 7, 1, d2[0].ptr, 7, 0, d2[1].ptr, N, 0, d2[2].ptr, N, 0, None)
+    L.qpms_hip_sync()
+    out2 = d2[2].to_array(np.complex128, (5, N))
+    ref2 = C2 - A2 @ B2
+    assert np.max(np.abs(out2 - ref2)) / np.max(np.abs(ref2)) < 1e-13
