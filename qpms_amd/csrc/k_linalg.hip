@@ -973,58 +973,71 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
 }
 
 // Triangular solves with few right-hand sides (nrhs <= TRV_R), in blocks of
-// TRV_B rows, on the permuted right-hand side W (n x nrhs, row-major):
-//   k_trsv_diag: one workgroup per matrix solves the w x w diagonal block
-//                (unit lower, forward; or upper, backward), thread i owning
-//                row r0 + i; per step the solved row is published through LDS
-//                (two slots alternate, one barrier per step) and every other
-//                row of the block subtracts its column entry times it;
+// TRV_W rows, on the permuted right-hand side W (n x nrhs, row-major):
+//   k_trsv_wave: one wave per matrix solves the w x w (w <= 64) diagonal
+//                block (unit lower, forward; or upper, backward), lane i
+//                owning row r0 + i, with no barrier per step: step s
+//                finalises x_j (j = s forward, w - 1 - s backward, where it
+//                takes the diagonal's reciprocal computed up front), every
+//                lane reads it with readlane, and the rows still open
+//                subtract their column entry times it; the block sits in LDS
+//                column-major (lane i reads entry (i, j) at j TRV_W + i, no
+//                bank conflict) and each lane loads its entries PF steps
+//                ahead;
 //   k_trsv_update: the rows outside the block (below it forward, above it
 //                backward) subtract the block's columns times its solution,
 //                one thread per row walking its row contiguously.
-constexpr int TRV_B = 256, TRV_R = 4;
+constexpr int TRV_R = 4, TRV_W = 64;
+__device__ __forceinline__ float trv_readlane(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double trv_readlane(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j), hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 template <class T, bool FWD>
-__global__ __launch_bounds__(TRV_B) void k_trsv_diag(int r0, int w, int nrhs, const typename C2<T>::t *LU, long lda,
+__global__ __launch_bounds__(TRV_W) void k_trsv_wave(int r0, int w, int nrhs, const typename C2<T>::t *LU, long lda,
                                                      long sA, typename C2<T>::t *W, long sW) {
   typedef typename C2<T>::t cT;
-  constexpr int PF = 4;   // row entries loaded this many steps ahead (off the per-step chain)
-  __shared__ cT sx[2][TRV_R];
+  constexpr int PF = 4;
+  __shared__ cT sL[TRV_W * TRV_W];   // sL[j TRV_W + i] = LU(r0 + i, r0 + j)
   LU += (long)blockIdx.z * sA;
   W += (long)blockIdx.z * sW;
   const int i = threadIdx.x;
   const bool mine = i < w;
+  if (mine) {
+    const cT *row = LU + (long)(r0 + i) * lda + r0;
+    for (int j = 0; j < w; ++j) sL[j * TRV_W + i] = row[j];
+  }
   cT y[TRV_R];
 #pragma unroll
   for (int c = 0; c < TRV_R; ++c) y[c] = (mine && c < nrhs) ? W[(long)(r0 + i) * nrhs + c] : cT{0, 0};
-  const cT *row = LU + (long)(r0 + (mine ? i : 0)) * lda + r0;
-  // backward: each row's own diagonal reciprocal, computed up front
-  const cT dinv = (!FWD && mine) ? cdiv<T>(cT{1, 0}, row[i]) : cT{1, 0};
+  __syncthreads();
+  const cT dinv = (!FWD && mine) ? cdiv<T>(cT{1, 0}, sL[i * TRV_W + i]) : cT{1, 0};
   auto col = [&](int s) { return FWD ? s : w - 1 - s; };
   cT buf[PF];
 #pragma unroll
-  for (int k = 0; k < PF; ++k) buf[k] = (mine && k < w) ? row[col(k)] : cT{0, 0};
+  for (int k = 0; k < PF; ++k) buf[k] = (mine && k < w) ? sL[col(k) * TRV_W + i] : cT{0, 0};
   for (int s0 = 0; s0 < w; s0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       const int s = s0 + k;
       if (s >= w) continue;   // uniform
-      const int j = col(s), par = s & 1;
+      const int j = col(s);
       const cT l = buf[k];
-      buf[k] = (mine && s + PF < w) ? row[col(s + PF)] : cT{0, 0};
-      if (i == j) {
+      buf[k] = (mine && s + PF < w) ? sL[col(s + PF) * TRV_W + i] : cT{0, 0};
+      if (!FWD && i == j) {
 #pragma unroll
-        for (int c = 0; c < TRV_R; ++c) {
-          if (!FWD) y[c] = cmul<T>(y[c], dinv);
-          sx[par][c] = y[c];
-        }
+        for (int c = 0; c < TRV_R; ++c) y[c] = cmul<T>(y[c], dinv);
       }
-      __syncthreads();
+      cT x[TRV_R];
+#pragma unroll
+      for (int c = 0; c < TRV_R; ++c) x[c] = cT{trv_readlane(y[c].x, j), trv_readlane(y[c].y, j)};
       if (mine && (FWD ? i > j : i < j)) {
 #pragma unroll
-        for (int c = 0; c < TRV_R; ++c) {
-          const cT x = sx[par][c];
-          y[c] = cT{y[c].x - (l.x * x.x - l.y * x.y), y[c].y - (l.x * x.y + l.y * x.x)};
-        }
+        for (int c = 0; c < TRV_R; ++c)
+          y[c] = cT{y[c].x - (l.x * x[c].x - l.y * x[c].y), y[c].y - (l.x * x[c].y + l.y * x[c].x)};
       }
     }
   }
@@ -1039,7 +1052,7 @@ __global__ __launch_bounds__(64) void k_trsv_update(int i0, int nrows, int r0, i
                                                     const typename C2<T>::t *LU, long lda, long sA,
                                                     typename C2<T>::t *W, long sW) {
   typedef typename C2<T>::t cT;
-  __shared__ cT sx[TRV_B * TRV_R];
+  __shared__ cT sx[TRV_W * TRV_R];
   LU += (long)blockIdx.z * sA;
   W += (long)blockIdx.z * sW;
   for (int e = threadIdx.x; e < w * nrhs; e += 64) sx[(e / nrhs) * TRV_R + e % nrhs] = W[(long)r0 * nrhs + e];
@@ -1079,17 +1092,17 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
   hipLaunchKernelGGL(k_gather_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs, perm, sP,
                      (const cT *)B, ldb, sB, work, (long)nrhs, sW);
   if (nrhs <= TRV_R) {
-    const int nblk = (n + TRV_B - 1) / TRV_B;
+    const int nblk = (n + TRV_W - 1) / TRV_W;
     for (int bi = 0; bi < nblk; ++bi) {
-      const int r0 = bi * TRV_B, w = min(TRV_B, n - r0), rem = n - r0 - w;
-      hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1, 1, batch), dim3(TRV_B), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      const int r0 = bi * TRV_W, w = min(TRV_W, n - r0), rem = n - r0 - w;
+      hipLaunchKernelGGL((k_trsv_wave<T, true>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
       if (rem > 0)
         hipLaunchKernelGGL(k_trsv_update<T>, dim3((rem + 63) / 64, 1, batch), dim3(64), 0, s, r0 + w, rem, r0, w, nrhs,
                            LU, lda, sA, work, sW);
     }
     for (int bi = nblk - 1; bi >= 0; --bi) {
-      const int r0 = bi * TRV_B, w = min(TRV_B, n - r0);
-      hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1, 1, batch), dim3(TRV_B), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      const int r0 = bi * TRV_W, w = min(TRV_W, n - r0);
+      hipLaunchKernelGGL((k_trsv_wave<T, false>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
       if (r0 > 0)
         hipLaunchKernelGGL(k_trsv_update<T>, dim3((r0 + 63) / 64, 1, batch), dim3(64), 0, s, 0, r0, r0, w, nrhs, LU, lda,
                            sA, work, sW);
