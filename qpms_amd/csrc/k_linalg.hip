@@ -722,8 +722,13 @@ __global__ __launch_bounds__(256) void k_tslu_l21(int n, int c0, int w, typename
   typedef typename C2<T>::t cT;
   constexpr int PW = PanelW<T>::W;
   __shared__ cT U[PW * PW];
+  __shared__ cT Dinv[PW];   // reciprocals of U's diagonal (1 where it is zero: the row entry is kept)
   A += (long)blockIdx.z * sA;
   for (int e = threadIdx.x; e < w * w; e += 256) U[(e / w) * PW + e % w] = A[(long)(c0 + e / w) * lda + c0 + e % w];
+  if (threadIdx.x < w) {
+    const cT d = A[(long)(c0 + threadIdx.x) * lda + c0 + threadIdx.x];
+    Dinv[threadIdx.x] = (d.x == 0 && d.y == 0) ? cT{1, 0} : cdiv<T>(cT{1, 0}, d);
+  }
   __syncthreads();
   const int r = c0 + w + blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
@@ -738,8 +743,7 @@ __global__ __launch_bounds__(256) void k_tslu_l21(int n, int c0, int w, typename
         const cT u = U[k * PW + i];
         v = cT{v.x - (x[k].x * u.x - x[k].y * u.y), v.y - (x[k].x * u.y + x[k].y * u.x)};
       }
-      const cT d = U[i * PW + i];
-      x[i] = (d.x == 0 && d.y == 0) ? v : cdiv<T>(v, d);
+      x[i] = cmul<T>(v, Dinv[i]);
       row[i] = x[i];
     }
   }
