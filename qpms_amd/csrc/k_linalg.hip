@@ -494,6 +494,21 @@ __device__ __forceinline__ void ts_axpy(v2d *__restrict__ Y, const v2d *__restri
   }
 }
 
+// 1 / p for the pivot: in FP32 as conj(p) / |p|^2 with the hardware
+// reciprocal (one rounding more than a division, off the chain's three
+// dependent IEEE divisions) while |p|^2 is comfortably normal, else by
+// Smith's division; FP64 always divides.
+template <class T> __device__ __forceinline__ typename C2<T>::t ts_crcp(typename C2<T>::t p);
+template <> __device__ __forceinline__ v2f ts_crcp<float>(v2f p) {
+  const float d = p.x * p.x + p.y * p.y;
+  if (d > 1e-30f && d < 1e30f) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return v2f{p.x * r, -p.y * r};
+  }
+  return cdiv<float>(v2f{1, 0}, p);
+}
+template <> __device__ __forceinline__ v2d ts_crcp<double>(v2d p) { return cdiv<double>(v2d{1, 0}, p); }
+
 template <class T>
 __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsShared<T> &sh) {
   typedef typename C2<T>::t cT;
@@ -521,7 +536,7 @@ __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsS
     if (live && nonzero) {
       const cT *P = X + pr * XS;
       cT *Y = X + t * XS;
-      const cT l = cmul<T>(xj, cdiv<T>(cT{1, 0}, P[j]));
+      const cT l = cmul<T>(xj, ts_crcp<T>(P[j]));
       Y[j] = l;
       ts_axpy(Y, P, l, j + 1, w);
     }
