@@ -494,10 +494,10 @@ __device__ __forceinline__ void ts_axpy(v2d *__restrict__ Y, const v2d *__restri
   }
 }
 
-// 1 / p for the pivot: in FP32 as conj(p) / |p|^2 with the hardware
-// reciprocal (one rounding more than a division, off the chain's three
-// dependent IEEE divisions) while |p|^2 is comfortably normal, else by
-// Smith's division; FP64 always divides.
+// 1 / p for the pivot as conj(p) / |p|^2 while |p|^2 is comfortably normal
+// (FP32: the hardware reciprocal, one rounding more than a division; FP64: one
+// IEEE division), instead of Smith's three dependent divisions, which remain
+// for tiny or huge pivots.
 template <class T> __device__ __forceinline__ typename C2<T>::t ts_crcp(typename C2<T>::t p);
 template <> __device__ __forceinline__ v2f ts_crcp<float>(v2f p) {
   const float d = p.x * p.x + p.y * p.y;
@@ -507,7 +507,14 @@ template <> __device__ __forceinline__ v2f ts_crcp<float>(v2f p) {
   }
   return cdiv<float>(v2f{1, 0}, p);
 }
-template <> __device__ __forceinline__ v2d ts_crcp<double>(v2d p) { return cdiv<double>(v2d{1, 0}, p); }
+template <> __device__ __forceinline__ v2d ts_crcp<double>(v2d p) {
+  const double d = p.x * p.x + p.y * p.y;
+  if (d > 1e-280 && d < 1e280) {
+    const double r = 1.0 / d;
+    return v2d{p.x * r, -p.y * r};
+  }
+  return cdiv<double>(v2d{1, 0}, p);
+}
 
 template <class T>
 __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsShared<T> &sh) {
