@@ -331,10 +331,11 @@ def bench_ewald(L, ev, stream):
                 for sh in ((0, 0), (s, 0), (-s, 0)):
                     tri.append([beta[0], beta[1], 0, k, 0, sh[0], sh[1], 0])
     tri = np.array(tri)
-    R = ec.xyweb(ec.B1, ec.B2, 0, True, 12.5 * ec.H_R, False)
-    R3 = np.concatenate([R, np.zeros((len(R), 1))], axis=1)
-    K = ec.K_points()
     Le = ec.lib_ewald()
+    # the point sets from the library's own xyWeb generator (PGen_xyWeb_*)
+    R = lattice_points(Le, ec.B1, ec.B2, 12.5 * ec.H_R)
+    R3 = np.concatenate([R, np.zeros((len(R), 1))], axis=1)
+    K = lattice_points(Le, ec.RB1, ec.RB2, 30 * 2 * np.pi / ec.A_LAT)
     e3c = Le.qpms_ewald3_constants_init(7, -1)
     n = len(tri)
     dtr, dR, dK = (qpms_amd.DeviceBuffer.from_array(x) for x in (tri, R3, K))
@@ -351,26 +352,59 @@ def bench_ewald(L, ev, stream):
                                            None)
         ev.record(a1, stream)
         times[part] = ev.elapsed_ms(a0, a1)
-    # parity on a sample of triples against the oracle
     ss = dss.to_array(np.complex128, (n, nsc))
     sl = dsl.to_array(np.complex128, (n, nsc))
-    err = 0.0
-    Rs = R[1:]
-    for t in range(0, n, n // 12):
-        k, beta, sh = tri[t, 3], tri[t, :2], tri[t, 5:7]
-        rr = Rs if (sh == 0).all() else R
-        o_s = ec.oracle_sigma_short(7, k, beta, sh, rr)
-        o_l = ec.oracle_sigma_long(7, k, beta, sh, K)
-        err = max(err, np.max(np.abs(ss[t] - o_s)) / np.max(np.abs(o_s)),
-                  np.max(np.abs(sl[t] - o_l)) / np.max(np.abs(o_l)))
     Le.qpms_ewald3_constants_free(e3c)
+    cpu = ewald_cpu_baseline(ec, tri, ss, sl, R, K)
     tot = times["short"] + times["long"]
     return {"metric": "Ewald sigma_short + sigma_long triples/s (FP64)", "value": n / (tot * 1e-3),
             "unit": "(beta, omega, shift) triples/s", "triples": n, "R_points": len(R), "K_points": len(K),
             "short_ms": times["short"], "long_ms": times["long"],
             "short_ns_per_point_pair": times["short"] * 1e6 / (n * len(R)),
             "long_ns_per_point_pair": times["long"] * 1e6 / (n * len(K)),
-            "parity_max_rel_err_sample": err}
+            "cpu_baseline": cpu}
+
+
+def lattice_points(Le, b1, b2, maxR):
+    """xyWeb points of the lattice (b1, b2) inside maxR (origin included,
+    strict cut), from the library's PGen_xyWeb generator."""
+    import qpms_amd
+    if not hasattr(Le, "_pgen_bound"):
+        import ewald_common as ec
+        Le.PGen_next_cart2.restype = type("PGenCart2", (ctypes.Structure,), {
+            "_fields_": [("flags", ctypes.c_int), ("x", ctypes.c_double), ("y", ctypes.c_double)]})
+        Le.PGen_next_cart2.argtypes = [ctypes.POINTER(ec.PGen)]
+        Le._pgen_bound = True
+    g = Le.PGen_xyWeb_new(qpms_amd.cart2_t(*b1), qpms_amd.cart2_t(*b2), 1e-13, qpms_amd.cart2_t(0, 0), 0, True, maxR,
+                          False)
+    pts = []
+    while True:
+        r = Le.PGen_next_cart2(ctypes.byref(g))
+        if not (r.flags & 2):   # PGEN_NOTDONE
+            break
+        pts.append((r.x, r.y))
+    return np.array(pts)
+
+
+def ewald_cpu_baseline(ec, tri, ss, sl, R, K):
+    """The oracle's sigma_short + sigma_long on 12 triples spread over the
+    workload, one host thread: its rate, and the largest relative difference
+    of the GPU's sums on those triples."""
+    n = len(tri)
+    err, cnt = 0.0, 0
+    t0 = time.perf_counter()
+    for t in range(0, n, n // 12):
+        k, beta, sh = tri[t, 3], tri[t, :2], tri[t, 5:7]
+        rr = R[1:] if (sh == 0).all() else R   # the origin carries no term for s = 0
+        o_s = ec.oracle_sigma_short(7, k, beta, sh, rr)
+        o_l = ec.oracle_sigma_long(7, k, beta, sh, K)
+        err = max(err, np.max(np.abs(ss[t] - o_s)) / np.max(np.abs(o_s)),
+                  np.max(np.abs(sl[t] - o_l)) / np.max(np.abs(o_l)))
+        cnt += 1
+    t_cpu = time.perf_counter() - t0
+    return {"value": cnt / t_cpu, "unit": "(beta, omega, shift) triples/s", "cores": 1, "kind": "port",
+            "sample": f"oracle sigma_short + sigma_long on {cnt} triples spread over the workload ({t_cpu:.1f} s)",
+            "max_rel_diff_gpu_vs_oracle": err}
 
 
 def bench_field(L, ev, stream):
@@ -378,8 +412,6 @@ def bench_field(L, ev, stream):
     lMax 3, seeded random outgoing-wave coefficients) on a 256 x 256 grid of
     points 200 nm above it, FP64; one k_scattered_e launch per map."""
     import qpms_amd
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ffi as of
     lMax, ne = 3, 15
     g = (np.arange(32) - 15.5) * 400e-9
     pos = np.array([(x, y, 0.0) for x in g for y in g])
@@ -407,18 +439,25 @@ def bench_field(L, ev, stream):
     ev.record(b, stream)
     ms = ev.elapsed_ms(a, b) / reps
     got = do.to_array(np.complex128, (len(pts), 3))
-    idx = rng.choice(len(pts), 48, replace=False)
-    t0 = time.perf_counter()
-    ref = of.scattered_E(lMax, pos, k, cv.ravel(), pts[idx])
-    t_cpu = time.perf_counter() - t0
-    err = float(np.max(np.abs(got[idx] - ref)) / np.max(np.abs(ref)))
     for x in (dp, dc, dw, do):
         x.free()
     pairs = len(pts) * len(pos)
     return {"metric": "scattered-field (point, particle) pairs/s (FP64)", "value": pairs / (ms * 1e-3),
             "unit": "pairs/s", "points": len(pts), "particles": len(pos), "lMax": lMax, "ms_per_map": ms,
-            "max_rel_diff_vs_oracle": err, "oracle_sample_points": len(idx),
-            "cpu_oracle_pairs_per_s_1thread": len(idx) * len(pos) / t_cpu}
+            "cpu_baseline": field_cpu_baseline(lMax, pos, k, cv, pts, got, rng.choice(len(pts), 48, replace=False))}
+
+
+def field_cpu_baseline(lMax, pos, k, cv, pts, got, idx):
+    """The oracle's field sum (long double, one host thread) at the sampled
+    points: its rate, and the largest relative difference of the GPU's map there."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as of
+    t0 = time.perf_counter()
+    ref = of.scattered_E(lMax, pos, k, cv.ravel(), pts[idx])
+    t_cpu = time.perf_counter() - t0
+    return {"value": len(idx) * len(pos) / t_cpu, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle field sum at {len(idx)} of the map's points ({t_cpu:.1f} s)",
+            "max_rel_diff_gpu_vs_oracle": float(np.max(np.abs(got[idx] - ref)) / np.max(np.abs(ref)))}
 
 
 def bench_beyn(L, window=None):
