@@ -3,32 +3,35 @@
 of the batched mixed-precision serving loop against the plain FP64 drop-in
 path on the same requests.
 
-Workload (the serving configuration the headline metric is quoted on): the
-seeded request stream of qpms_amd.serve.serving_stream — Poisson arrivals,
-clusters of N particles (log-uniform 16..1024) on a jittered square lattice,
-lMax in {2,3,4}, random spheres and frequencies; 70 % plane-wave solves,
-20 % 2D-periodic determinants, 10 % 32-node Beyn contours. Its full 10,000
-requests are hours of GPU work (one 32-node contour at N ~ 1000, lMax 4 is
-32 LUs of order ~45,000), so a step replays the stream in real time over a
-fixed window: requests are submitted at their arrival times, the loop runs,
-and at the end of the window queued requests are cancelled and contours in
-flight stop after their current node chunk. `value` counts the requests
-finished inside the window per second of window.
+Workload: the seeded 10,000-request stream of qpms_amd.serve.serving_stream
+(Poisson arrivals, clusters of N particles log-uniform in 16..1024 on a
+jittered square lattice, lMax in {2,3,4}, random spheres and frequencies;
+70 % plane-wave solves, 20 % 2D-periodic determinants, 10 % 32-node Beyn
+contours). A step is one slice of `--slice` consecutive requests; the timed
+region is the first steps x slice requests of the stream, replayed at their
+arrival times into one serving loop and run until every request is answered
+(nothing is cancelled or skipped). `value` = requests answered / wall time of
+that replay; latency percentiles are over every request of it (arrival to
+result, queueing included). The warm-up steps replay the requests that follow
+the timed ones.
 
 The plain FP64 path (qpms_hip_request_plain: the public scattering-system API
-one call after another, host buffers, FP64 LU) then answers the same requests
-the serving loop finished, one at a time; its rate is requests per second of
-its own busy time. Every served result is compared with the plain one
-(extinction, log-determinant, Beyn singular values) and the largest relative
-difference is reported.
+one call after another, host buffers, FP64 LU) answers the timed requests one
+at a time in stream order within a time budget, never starting a request its
+size says cannot finish inside the budget; the serving loop then replays
+exactly the requests the plain path covered, so `vs_plain_fp64` compares the
+two paths on identical work. Every plain result is compared with the served
+one (extinction, log-determinant, Beyn singular values and eigenvalues).
 
-`roofline`: the dominant work of the loop is the FP32 LU of the mode problem
-(tournament-pivoted panels + trailing complex GEMMs on MFMA); it is timed
-live with HIP events on the library stream on a mode-problem matrix of order
-16380 and priced at 8/3 n^3 real flops against the FP32 matrix peak.
-`w_build`: the plain translation-matrix build (61440^2 complex128, HBM
-roofline). `cpu_baseline`: the C oracle's full pipeline (W, M = I - T W, LU,
-extinction) on host threads for the first plane-wave requests of the stream.
+`roofline`: the FP32 LU of a mode-problem matrix of order 16380 (the loop's
+dominant work: tournament-pivoted panels plus trailing complex GEMMs on
+MFMA), timed live with HIP events on the library stream and priced at
+8/3 n^3 real flops against the FP32 matrix peak; `traffic` comes from the
+committed rocprofv3 counter summary of the same probe. `w_build`: the plain
+translation-matrix build (61440^2 complex128, HBM roofline). `cpu_baseline`:
+the reference's CPU path of the plane-wave solves of the timed requests (the
+oracle's pair loop for M = I - T W, host LAPACK zgetrf/zgetrs as
+scatsystem.c:2214/2270 call it), all host threads and one thread.
 """
 import argparse
 import ctypes
@@ -47,9 +50,19 @@ C0 = 2.99792458e8
 METRIC = "mode-problem solves/s + p99 latency at 1e-9 rel err, batched-mixed vs plain FP64"
 HBM_PEAK = 8.0e12
 FP32_MFMA_PEAK = 157.3e12
-# HBM bytes per k_w_build<3,false> launch at N = 2048 (rocprofv3 FETCH_SIZE x 2.000
-# + WRITE_SIZE x 1.000, each scaled by a 2 GiB copy kernel; profiles/k_w_build_*)
-W_BUILD_TRAFFIC = 60.436e9
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profile_traffic(name):
+    """HBM bytes per launch recorded in profiles/<name>.md by its counter
+    passes (the line `traffic_bytes_per_launch: <bytes>`), or None."""
+    try:
+        for ln in open(os.path.join(PROFILES, name + ".md")):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -76,24 +89,10 @@ class HipEvents:
         return ms.value
 
 
-def cluster_positions(n=2048, box=(4e-6, 4e-6, 0.5e-6), dmin=120e-9, seed=20261202):
-    rng = np.random.default_rng(seed)
-    pts = np.empty((n, 3))
-    cnt = 0
-    while cnt < n:
-        cand = rng.uniform(0.0, 1.0, (4096, 3)) * np.asarray(box)
-        for p in cand:
-            if cnt == 0 or np.min(np.sum((pts[:cnt] - p) ** 2, axis=1)) >= dmin * dmin:
-                pts[cnt] = p
-                cnt += 1
-                if cnt == n:
-                    break
-    return pts
-
-
 def measure_w_build(L, ev, stream, nfreq=8):
     """Translation-matrix build, N = 2048, lMax 3, FP64: HBM roofline."""
     import qpms_amd
+    from qpms_amd.workloads import cluster_positions
     pos = cluster_positions()
     calc = qpms_amd.TransCalculator(3)
     Ldim = 30 * len(pos)
@@ -116,7 +115,31 @@ def measure_w_build(L, ev, stream, nfreq=8):
     ach = nbytes / (avg * 1e-3)
     return {"kernel": "k_w_build<3,false>", "workload": "W of 2048 particles, lMax 3 (61440^2 complex128), FP64",
             "bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-            "traffic": W_BUILD_TRAFFIC, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": nbytes}
+            "traffic": profile_traffic("k_w_build_translation_2048p_lmax3"), "avg_launch_ms": avg,
+            "algorithmic_bytes_per_launch": nbytes}
+
+
+def w_probe(L):
+    """Counter-pass probe: one plain copy kernel over 2 GiB (k_copy2d reads
+    and writes 2 GiB: the FETCH_SIZE / WRITE_SIZE calibration), then one
+    translation-matrix build of the N = 2048, lMax 3 workload."""
+    import qpms_amd
+    from qpms_amd.workloads import cluster_positions
+    n = 1 << 27
+    a, b = qpms_amd.DeviceBuffer(16 * n), qpms_amd.DeviceBuffer(16 * n)
+    L.qh_copy_c64.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.qh_copy_c64(n, a.ptr, b.ptr, None)
+    L.qpms_hip_sync()
+    a.free()
+    b.free()
+    pos = cluster_positions()
+    calc = qpms_amd.TransCalculator(3)
+    Ld = 30 * len(pos)
+    dpos, dW = qpms_amd.DeviceBuffer.from_array(pos), qpms_amd.DeviceBuffer(16 * Ld * Ld)
+    L.qpms_hip_build_translation_matrix_dev(calc.c, len(pos), dpos.ptr, qpms_amd.c_cplx.of(1.52 * 2.0 * EV / C0), 3,
+                                            dW.ptr, None)
+    L.qpms_hip_sync()
+    print(json.dumps({"w_probe": "done", "copy_bytes_each_way": 16 * n, "w_bytes": 16 * Ld * Ld}), flush=True)
 
 
 def measure_peaks(L, ev, stream):
@@ -181,9 +204,6 @@ def measure_peaks(L, ev, stream):
     return out
 
 
-LU_TRAFFIC_BYTES = 2.0774e11
-
-
 def measure_lu(L, ev, stream, n_part=546, reps=3):
     """FP32 LU of a mode-problem matrix of order 30 * n_part = 16380 (the
     serving loop's dominant work), HIP events on the library stream."""
@@ -223,51 +243,43 @@ def measure_lu(L, ev, stream, n_part=546, reps=3):
     ach = flops / (avg * 1e-3)
     return {"bound": "mfma", "achieved": ach / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": ach / FP32_MFMA_PEAK,
-            # HBM bytes per factorisation from two rocprofv3 --pmc passes over this
-            # probe (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, copy-kernel calibration;
-            # profiles/lu_fp32_order16380.md)
-            "traffic": LU_TRAFFIC_BYTES,
+            # HBM bytes per factorisation from the two rocprofv3 --pmc passes of
+            # this probe (FETCH_SIZE, WRITE_SIZE, copy-kernel calibration)
+            "traffic": profile_traffic("lu_fp32_order16380"),
             "kernel": "FP32 LU, order 16380 (tournament panels + trailing cgemm)",
             "avg_launch_ms": avg, "algorithmic_flops_per_launch": flops}
 
 
-def replay(stream_reqs, window, mixed=True, arena_gib=160, drain_cancel=True):
-    """Submit requests at their arrival times for `window` seconds; returns
-    (responses by index, requests submitted, wall seconds until drained)."""
-    from qpms_amd import serve as sv
-    srv = sv.Server(arena_gib=arena_gib, nstreams=8, mixed=mixed)
-    id2i = {}
-    got = {}
+def meminfo_gib():
+    """(free, total) device memory in GiB (hipMemGetInfo)."""
+    h = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    assert h.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+    return f.value / 2 ** 30, t.value / 2 ** 30
+
+
+def replay(srv, reqs, label):
+    """Submit reqs at their arrival times (relative to the first one) and run
+    the loop until every request is answered. Returns ({index: (response,
+    seconds from the first submission)}, wall seconds)."""
+    t_first = reqs[0].arrival
+    id2i, got = {}, {}
     nxt = 0
-    if drain_cancel:   # start nothing that is not estimated to finish inside the window
-        srv.set_deadline(srv.clock() + window)
     t0 = time.perf_counter()
     last_log = t0
-    while True:
+    while len(got) < len(reqs):
         now = time.perf_counter() - t0
-        while nxt < len(stream_reqs) and stream_reqs[nxt].arrival <= now and now < window:
-            id2i[srv.submit(stream_reqs[nxt])] = nxt
+        while nxt < len(reqs) and reqs[nxt].arrival - t_first <= now:
+            id2i[srv.submit(reqs[nxt])] = nxt
             nxt += 1
-        if now >= window:
-            break
         srv.progress(wait=False)
         for r in srv.poll():
             got[id2i[r.id]] = (r, time.perf_counter() - t0)
-        if time.perf_counter() - last_log > 10:
-            log(f"serving t={now:.0f}s submitted {nxt} done {len(got)} pending {srv.pending()}")
+        if time.perf_counter() - last_log > 15:
+            log(f"{label}: t={now:.0f}s submitted {nxt}/{len(reqs)} answered {len(got)}")
             last_log = time.perf_counter()
         time.sleep(0.0005)
-    if drain_cancel:
-        srv.cancel()   # queued requests are answered at once (status -2)
-    while True:
-        for r in srv.poll():
-            got[id2i[r.id]] = (r, time.perf_counter() - t0)
-        if not srv.pending():
-            break
-        srv.progress(wait=True)
-    wall = time.perf_counter() - t0
-    srv.close()
-    return got, nxt, wall
+    return got, time.perf_counter() - t0
 
 
 def rel_diff(a, b, req):
@@ -279,31 +291,75 @@ def rel_diff(a, b, req):
     return sv.beyn_rel_diff(a, b, req.c.rank_tol)
 
 
-CPU_MAX_DIM = 3000
+def plain_estimate_s(req):
+    """Conservative seconds of one plain-path request, from its order: FP64
+    LUs at 30 TFLOP/s, the host round trips of the mode-problem matrix at
+    8 GB/s each way, and a fixed cost (used only to decide whether a request
+    can still finish inside the plain path's budget)."""
+    from qpms_amd import serve as sv
+    L = float(req.dim)
+    one = 8.0 / 3.0 * L ** 3 / 30e12 + 3 * 16.0 * L * L / 8e9 + 0.05
+    if req.kind == sv.KIND_BEYN:
+        return req.c.beyn_nodes * one
+    if req.kind == sv.KIND_PERIODIC:
+        return 2 * one + 0.5
+    return one
 
 
-def cpu_baseline_scatter(reqs, budget_s, threads):
-    """Oracle pipeline on the host for the plane-wave requests of order at
-    most CPU_MAX_DIM, in stream order, until budget_s of CPU time is used
-    (one request of the largest orders alone would take minutes)."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_scatter(reqs, served, budget_s):
+    """The reference's CPU path (oracle pair loop for M = I - T W, host LAPACK
+    zgetrf/zgetrs, extinction) on the plane-wave requests of `reqs` in stream
+    order, once on every host thread and once on one thread, each until
+    budget_s of wall time is used (whole requests only). Also the largest
+    relative difference of the served extinctions from it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
     from qpms_amd import serve as sv
+    lap = oracle_ffi.host_lapack()
     o = oracle_ffi.oracle()
-    o.or_set_threads(threads)
-    done, t_used, sigma = 0, 0.0, {}
-    for i, r in enumerate(reqs):
-        if r.kind != sv.KIND_SCATTER or r.dim > CPU_MAX_DIM:
-            continue
-        t0 = time.perf_counter()
-        res = oracle_ffi.finite_cluster(r.pos, r.c.lMax, r.c.radius, complex(r.c.eps_in.re, r.c.eps_in.im),
-                                        r.c.eps_bg, r.c.omega / EV, (0, 0, 1), np.array([1, 0, 0], complex))
-        t_used += time.perf_counter() - t0
-        sigma[i] = float(res["sigma_ext"])
-        done += 1
-        if t_used > budget_s:
-            break
-    return done, t_used, sigma
+    cores = min(16, os.cpu_count() or 1)
+    pw = [i for i, r in enumerate(reqs) if r.kind == sv.KIND_SCATTER]
+    out = {"unit": "plane-wave solves/s", "kind": "port", "cpu_model": cpu_model(), "lapack": lap.name}
+    diffs = []
+    for tag, threads in (("all", cores), ("one", 1)):
+        lap.set_threads(threads)
+        o.or_set_threads(threads)
+        done, t_used, dims = 0, 0.0, []
+        for i in pw:
+            r = reqs[i]
+            if t_used > 0 and t_used + 8.0 / 3.0 * float(r.dim) ** 3 / (5e9 * threads) > budget_s:
+                break   # this order alone would overrun the budget
+            t0 = time.perf_counter()
+            se = oracle_ffi.reference_cpu_solve(r.pos, r.c.lMax, r.c.radius, complex(r.c.eps_in.re, r.c.eps_in.im),
+                                                r.c.eps_bg, r.c.omega, (0, 0, 1), np.array([1, 0, 0], complex), lap)
+            t_used += time.perf_counter() - t0
+            done += 1
+            dims.append(r.dim)
+            if tag == "all" and i in served:
+                diffs.append(abs(served[i][0].sigma_ext - se) / abs(se))
+            if t_used > budget_s:
+                break
+        out[tag] = {"threads": threads, "requests": done, "seconds": t_used,
+                    "value": done / t_used if t_used > 0 else None, "orders": dims}
+        log(f"cpu baseline ({threads} threads): {done} plane-wave solves in {t_used:.1f} s")
+    lap.set_threads(cores)
+    out["value"] = out["all"]["value"]
+    out["cores"] = cores
+    out["sample"] = (f"the reference's CPU path on the first plane-wave requests of the timed segment in stream order "
+                     f"({out['all']['requests']} on {cores} threads, {out['one']['requests']} on 1 thread, "
+                     f"{budget_s:.0f} s budget each); the GPU value covers every request kind and order")
+    out["served_vs_cpu_max_rel_diff"] = float(max(diffs)) if diffs else None
+    return out
 
 
 def bench_ewald(L, ev, stream):
@@ -474,7 +530,7 @@ def bench_beyn(L, window=None):
                  rank_tol=sv.beyn_rank_tol(9216, 64, 0.05 * EV))
     out = {}
     for mixed in (True, False):
-        srv = sv.Server(arena_gib=160, mixed=mixed)
+        srv = sv.Server(arena_gib=100, mixed=mixed)
         t0 = time.perf_counter()
         rid = srv.submit(req)
         got = None
@@ -512,7 +568,7 @@ def per_kind(n, plain=True):
     from qpms_amd import serve as sv
     reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
             if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
-    srv = sv.Server(arena_gib=160, nstreams=8, mixed=True)
+    srv = sv.Server(arena_gib=150, nstreams=8, mixed=True)
     for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
         srv.submit(r)
     while srv.pending():
@@ -550,7 +606,7 @@ def burst(n):
     from qpms_amd import serve as sv
     reqs = [r.clone(0.0) for r in sv.serving_stream(n=10000, rate=10.0)
             if r.dim <= 25000 and not (r.kind == sv.KIND_BEYN and r.dim > 6000)][:n]
-    srv = sv.Server(arena_gib=160, nstreams=8, mixed=True)
+    srv = sv.Server(arena_gib=150, nstreams=8, mixed=True)
     for r in [x for x in reqs if x.dim < 2000][:6]:   # warm-up
         srv.submit(r)
     while srv.pending():
@@ -574,17 +630,22 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["serving", "ewald", "beyn", "peaks", "field"], default="serving")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--window", type=float, default=45.0, help="seconds of stream replayed per step")
-    ap.add_argument("--rate", type=float, default=25.0,
-                    help="request arrival rate (1/s); above the loop's capacity, so the window measures capacity")
+    ap.add_argument("--slice", type=int, default=2, help="requests of the stream per step")
+    ap.add_argument("--rate", type=float, default=2.0,
+                    help="request arrival rate (1/s); above the loop's capacity on this stream, so the replay "
+                         "measures capacity")
+    ap.add_argument("--arena-gib", type=float, default=150.0,
+                    help="serving arena (the rest of the device is left to the plain path's own allocations)")
     ap.add_argument("--plain-budget", type=float, default=150.0, help="seconds for the plain FP64 comparison")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per thread setting for the CPU leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-wbuild", action="store_true")
     ap.add_argument("--lu-probe", action="store_true", help="only the FP32 LU measurement (for rocprofv3)")
+    ap.add_argument("--w-probe", action="store_true",
+                    help="one 2 GiB copy kernel (counter calibration) and one N = 2048 W build (for rocprofv3 --pmc)")
     ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
     ap.add_argument("--burst", type=int, default=0, help="submit the per-kind requests at once and drain them")
     args = ap.parse_args()
@@ -601,6 +662,9 @@ def main():
     stream = L.qpms_hip_stream()
     ev = HipEvents()
 
+    if args.w_probe:
+        w_probe(L)
+        return
     if args.lu_probe:
         print(json.dumps(measure_lu(L, ev, stream, reps=2)), flush=True)
         return
@@ -631,97 +695,94 @@ def main():
     roof = measure_lu(L, ev, stream)
     log(f"FP32 LU n=16380: {roof['avg_launch_ms']:.1f} ms, {roof['achieved']:.1f} TF/s")
 
+    S, K, W = args.slice, args.steps, args.warmup
     reqs = sv.serving_stream(n=10000, rate=args.rate)
-    # warm-up: a few small requests of each kind (code objects, GEMM handles)
-    for _ in range(args.warmup):
-        small = [r.clone(0.0) for r in reqs[:200] if r.dim <= 2000][:12]
-        got, _, wall = replay(small, window=0.01, arena_gib=32, drain_cancel=False)
-        log(f"warm-up: {len(got)} requests in {wall:.1f} s")
+    timed, warm = reqs[:K * S], reqs[K * S:(K + W) * S]
+    f0, tot = meminfo_gib()
+    srv = sv.Server(arena_gib=args.arena_gib, nstreams=8, mixed=True)
+    log(f"device memory: {f0:.1f} of {tot:.1f} GiB free before the server; arena "
+        f"{srv.arena_bytes() / 2 ** 30:.0f} GiB; {meminfo_gib()[0]:.1f} GiB free after")
+    if warm:
+        got_w, wall_w = replay(srv, warm, "warm-up")
+        log(f"warm-up: {len(warm)} requests ({W} steps) in {wall_w:.1f} s")
 
-    served = {}
-    win_total, wall_total, submitted = 0.0, 0.0, 0
-    for step in range(args.steps):
-        got, nsub, wall = replay(reqs, args.window)
-        submitted += nsub
-        win_total += args.window
-        wall_total += wall
-        for i, (r, t) in got.items():
-            served.setdefault(i, (r, t))
-        log(f"step {step}: submitted {nsub}, answered {sum(1 for r, _ in got.values() if r.status == 0)}, "
-            f"wall {wall:.1f} s")
-    ok = {i: (r, t) for i, (r, t) in served.items() if r.status == 0 and t <= args.window}
-    late = {i: (r, t) for i, (r, t) in served.items() if r.status == 0 and t > args.window}
-    cancelled = sum(1 for r, _ in served.values() if r.status == -2)
-    lat = np.array([t - reqs[i].arrival for i, (r, t) in ok.items()]) if ok else np.array([np.nan])
-    value = len(ok) / win_total
-    kinds = {k: sum(1 for i in ok if reqs[i].kind == k) for k in (0, 1, 2)}
-    bsz = np.array([r.batch for r, _ in ok.values() if r.kind == sv.KIND_SCATTER] or [0])
-    log(f"plane-wave solves by batch size: " + ", ".join(f"{b}: {int(np.sum(bsz == b))}" for b in np.unique(bsz)))
-    log(f"served {len(ok)} in window ({kinds}), {len(late)} after, {cancelled} cancelled; "
-        f"p50 {np.percentile(lat, 50):.3f} s p99 {np.percentile(lat, 99):.3f} s")
+    L.qpms_hip_sync()
+    got, wall = replay(srv, timed, "timed")
+    L.qpms_hip_sync()
+    t_first = timed[0].arrival
+    lat = np.array([t - (timed[i].arrival - t_first) for i, (r, t) in got.items()])
+    failed = sorted(i for i, (r, _) in got.items() if r.status != 0)
+    value = len(timed) / wall
+    kinds = {str(k): sum(1 for r in timed if r.kind == k) for k in (0, 1, 2)}
+    step_done = [max(got[i][1] for i in range(k * S, (k + 1) * S)) for k in range(K)]
+    bsz = np.array([r.batch for r, _ in got.values() if r.kind == sv.KIND_SCATTER] or [0])
+    log("plane-wave solves by batch size: " + ", ".join(f"{b}: {int(np.sum(bsz == b))}" for b in np.unique(bsz)))
+    log(f"timed: {len(timed)} requests ({kinds}) answered in {wall:.1f} s = {value:.3f}/s; {len(failed)} failed; "
+        f"p50 {np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s")
 
     plain = None
-    if not args.no_plain and ok:
+    if not args.no_plain:
         times, errs, kinds_p = [], [], []
         t_start = time.perf_counter()
-        last_log = time.perf_counter()
-        # a seeded random order of the served requests, so that a budget that
-        # ends early still samples the whole served mix
-        order = [sorted(ok)[q] for q in np.random.default_rng(7).permutation(len(ok))]
-        for i in order:
-            if time.perf_counter() - last_log > 20:
-                log(f"plain FP64: {len(times)} requests done")
-                last_log = time.perf_counter()
-            t0 = time.perf_counter()
-            pr = sv.request_plain(reqs[i])
-            times.append(time.perf_counter() - t0)
-            errs.append(rel_diff(ok[i][0], pr, reqs[i]))
-            kinds_p.append(reqs[i].kind)
-            if time.perf_counter() - t_start > args.plain_budget:
+        stopped_at = None
+        for i, r in enumerate(timed):
+            left = args.plain_budget - (time.perf_counter() - t_start)
+            if plain_estimate_s(r) > left:
+                stopped_at = i
                 break
-        n_p = len(times)
-        plain_s_by_kind = {str(k): float(sum(t for t, kk in zip(times, kinds_p) if kk == k)) for k in (0, 1, 2)}
-        plain = {"solves_per_s": n_p / sum(times), "requests": n_p, "of_served": len(ok),
-                 "busy_s_by_kind": plain_s_by_kind,
-                 "p50_s": float(np.percentile(times, 50)), "p99_s": float(np.percentile(times, 99)),
-                 "max_rel_diff_served_vs_fp64": float(np.max(errs)),
-                 "max_rel_diff_by_kind": {str(k): float(max([e for e, kk in zip(errs, kinds_p) if kk == k] or [0]))
-                                          for k in (0, 1, 2)}}
-        log(f"plain FP64: {n_p} requests in {sum(times):.1f} s busy ({plain['solves_per_s']:.2f}/s); "
-            f"max rel diff {plain['max_rel_diff_served_vs_fp64']:.2e}")
+            log(f"plain FP64: request {i} kind {r.kind} order {r.dim}")
+            t0 = time.perf_counter()
+            pr = sv.request_plain(r)
+            times.append(time.perf_counter() - t0)
+            errs.append(rel_diff(got[i][0], pr, r))
+            kinds_p.append(r.kind)
+        P = len(times)
+        if P:
+            if P == len(timed):
+                served_wall, served_src = wall, "the timed replay"
+            else:   # the serving loop on exactly the plain path's requests
+                L.qpms_hip_sync()
+                got_p, served_wall = replay(srv, timed[:P], "served prefix")
+                L.qpms_hip_sync()
+                served_src = f"a replay of requests 0..{P - 1}"
+            plain = {"requests": P, "of_timed": len(timed), "plain_busy_s": float(sum(times)),
+                     "solves_per_s": P / sum(times), "served_wall_s": served_wall, "served_solves_per_s": P / served_wall,
+                     "served_measured_by": served_src,
+                     "stopped_before": None if stopped_at is None else
+                     {"request": stopped_at, "kind": timed[stopped_at].kind, "order": timed[stopped_at].dim,
+                      "estimated_plain_s": plain_estimate_s(timed[stopped_at])},
+                     "busy_s_by_kind": {str(k): float(sum(t for t, kk in zip(times, kinds_p) if kk == k))
+                                        for k in (0, 1, 2)},
+                     "p50_s": float(np.percentile(times, 50)), "p99_s": float(np.percentile(times, 99)),
+                     "max_rel_diff_served_vs_fp64": float(np.max(errs)),
+                     "max_rel_diff_by_kind": {str(k): float(max([e for e, kk in zip(errs, kinds_p) if kk == k] or [0]))
+                                              for k in (0, 1, 2)}}
+            log(f"plain FP64: {P} requests in {sum(times):.1f} s busy; served {served_wall:.1f} s on the same; "
+                f"max rel diff {plain['max_rel_diff_served_vs_fp64']:.2e}")
+    srv.close()
+    log(f"device memory after the server: {meminfo_gib()[0]:.1f} GiB free")
 
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        n_c, t_c, sig_o = cpu_baseline_scatter(reqs, args.cpu_budget, threads)
-        # the served answers of those requests against the oracle's
-        cmp = [abs(ok[i][0].sigma_ext - so) / abs(so) for i, so in sig_o.items() if i in ok]
-        cpu = {"value": n_c / t_c if t_c > 0 else None, "unit": "plane-wave solves/s", "cores": threads,
-               "kind": "port",
-               "served_vs_oracle": {"requests": len(cmp), "max_rel_diff": float(max(cmp)) if cmp else None},
-               "sample": f"oracle pipeline (W, I - T W, LU, extinction) on the first {n_c} plane-wave requests of "
-                         f"order <= {CPU_MAX_DIM} in the stream ({t_c:.1f} s); the GPU figure covers every order "
-                         f"and also determinants and contours"}
-        log(f"cpu baseline: {n_c} plane-wave solves in {t_c:.1f} s; served vs oracle on {len(cmp)} of them: "
-            f"max rel diff {max(cmp) if cmp else float('nan'):.2e}")
+    cpu = None if args.no_cpu else cpu_baseline_scatter(timed, got, args.cpu_budget)
 
     line = {
         "metric": METRIC,
         "value": value,
-        "unit": "requests/s finished within the window (plane-wave solves, periodic determinants, Beyn contours)",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall_total / args.steps * 1e3,
+        "unit": "requests/s (plane-wave solves, periodic determinants, Beyn contours; every request answered)",
+        "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 LU + c128 refinement (solves), c128 (builds, Ewald, determinants, Beyn SVD)",
+        "dtype": "c64 LU + c128 refinement (solves, contour nodes), c128 (builds, Ewald sums, determinants, Beyn SVD)",
         "data": "synthetic (seeded request stream)",
-        "config": {"workload": "serving stream: Poisson arrivals, N log-uniform 16..1024, lMax 2-4, "
-                               "70/20/10 plane-wave / periodic-determinant / 32-node Beyn",
-                   "rate_per_s": args.rate, "window_s": args.window, "requests_submitted": submitted,
-                   "served_in_window": len(ok), "served_by_kind": kinds, "served_after_window": len(late),
-                   "cancelled_at_window_end": cancelled},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "config": {"workload": f"serving stream: the first {K * S} requests (steps of {S}) of the seeded 10,000-request "
+                               "stream (Poisson arrivals, N log-uniform 16..1024, lMax 2-4, 70/20/10 plane-wave / "
+                               "periodic-determinant / 32-node Beyn), run to completion",
+                   "rate_per_s": args.rate, "requests": len(timed), "by_kind": kinds,
+                   "max_order": int(max(r.dim for r in timed)), "failed": len(failed),
+                   "warmup_requests": len(warm), "step_done_s": [round(x, 3) for x in step_done]},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(np.max(lat))},
         "plain_fp64": plain,
-        "vs_plain_fp64": (value / plain["solves_per_s"]) if plain else None,
+        "vs_plain_fp64": (plain["plain_busy_s"] / plain["served_wall_s"]) if plain else None,
         "roofline": roof,
         "w_build": wb,
         "measured_peaks": peaks,

@@ -33,27 +33,27 @@ extern "C" {
 /* Basic types — qpms/qpms_types.h                                           */
 /* ------------------------------------------------------------------------ */
 
-typedef int qpms_lm_t;              /* qpms_types.h:47 */
-typedef int qpms_l_t;               /* qpms_types.h:49 */
-typedef qpms_lm_t qpms_m_t;         /* qpms_types.h:53 */
-typedef size_t qpms_y_t;            /* qpms_types.h:69 */
-typedef size_t qpms_y_sc_t;         /* qpms_types.h:85 */
-typedef unsigned long long qpms_uvswfi_t; /* qpms_types.h:108 */
-typedef int32_t qpms_ss_tmi_t;      /* qpms_types.h:329 */
-typedef int32_t qpms_ss_tmgi_t;     /* qpms_types.h:331 */
-typedef int32_t qpms_ss_pi_t;       /* qpms_types.h:333 */
-typedef int qpms_gmi_t;             /* qpms_types.h:337 */
-typedef int qpms_iri_t;             /* qpms_types.h:339 */
+typedef int qpms_lm_t;              /* qpms_types.h:25 */
+typedef int qpms_l_t;               /* qpms_types.h:27 */
+typedef qpms_lm_t qpms_m_t;         /* qpms_types.h:31 */
+typedef size_t qpms_y_t;            /* qpms_types.h:44 */
+typedef size_t qpms_y_sc_t;         /* qpms_types.h:59 */
+typedef unsigned long long qpms_uvswfi_t; /* qpms_types.h:81 */
+typedef int32_t qpms_ss_tmi_t;      /* qpms_types.h:310 */
+typedef int32_t qpms_ss_tmgi_t;     /* qpms_types.h:313 */
+typedef int32_t qpms_ss_pi_t;       /* qpms_types.h:316 */
+typedef int qpms_gmi_t;             /* qpms_types.h:320 */
+typedef int qpms_iri_t;             /* qpms_types.h:323 */
 #define QPMS_NO_IRREP ((qpms_iri_t) -1)
 
-/* qpms_types.h:88-93 */
+/* qpms_types.h:62-66 */
 typedef enum {
 	QPMS_VSWF_ELECTRIC = 2,
 	QPMS_VSWF_MAGNETIC = 1,
 	QPMS_VSWF_LONGITUDINAL = 0
 } qpms_vswf_type_t;
 
-/* qpms_types.h:112-119 */
+/* qpms_types.h:85-92 */
 typedef enum {
 	QPMS_SUCCESS = 0,
 	QPMS_ERROR = -1,
@@ -63,7 +63,7 @@ typedef enum {
 	QPMS_NAN_ENCOUNTERED = 1024
 } qpms_errno_t;
 
-/* qpms_types.h:128-196 (values must match bit for bit) */
+/* qpms_types.h:104-168 (values must match bit for bit) */
 typedef enum {
 	QPMS_NORMALISATION_UNDEF = 0,
 	QPMS_NORMALISATION_INVERSE = 1,
@@ -87,7 +87,7 @@ typedef enum {
 	QPMS_NORMALISATION_DEFAULT = 65536 | 8
 } qpms_normalisation_t;
 
-/* qpms_types.h:204-210 */
+/* qpms_types.h:176-182 */
 typedef enum {
 	QPMS_BESSEL_REGULAR = 1,
 	QPMS_BESSEL_SINGULAR = 2,
@@ -96,20 +96,20 @@ typedef enum {
 	QPMS_BESSEL_UNDEF = 0
 } qpms_bessel_t;
 
-typedef struct cart3_t { double x, y, z; } cart3_t;            /* qpms_types.h:214 */
-typedef struct cart2_t { double x, y; } cart2_t;                /* qpms_types.h:230 */
-typedef struct sph_t { double r, theta, phi; } sph_t;           /* qpms_types.h:235 */
-typedef struct csph_t { _Complex double r; double theta, phi; } csph_t; /* qpms_types.h:240 */
-typedef struct pol_t { double r, phi; } pol_t;                  /* qpms_types.h:251 */
-typedef struct ccart3_t { _Complex double x, y, z; } ccart3_t;  /* qpms_types.h:219 */
+typedef struct cart3_t { double x, y, z; } cart3_t;            /* qpms_types.h:186 */
+typedef struct cart2_t { double x, y; } cart2_t;                /* qpms_types.h:202 */
+typedef struct sph_t { double r, theta, phi; } sph_t;           /* qpms_types.h:207 */
+typedef struct csph_t { _Complex double r; double theta, phi; } csph_t; /* qpms_types.h:212 */
+typedef struct pol_t { double r, phi; } pol_t;                  /* qpms_types.h:223 */
+typedef struct ccart3_t { _Complex double x, y, z; } ccart3_t;  /* qpms_types.h:191 */
 typedef struct csphvec_t { _Complex double rc, thetac, phic; } csphvec_t; /* qpms_types.h:218 */
-typedef union anycoord_point_t {                               /* qpms_types.h:256 */
+typedef union anycoord_point_t {                               /* qpms_types.h:229 */
 	double z; cart3_t cart3; cart2_t cart2; sph_t sph; pol_t pol;
 } anycoord_point_t;
-typedef struct qpms_quat_t { _Complex double a, b; } qpms_quat_t; /* qpms_types.h:283 */
-typedef struct qpms_irot3_t { qpms_quat_t rot; short det; } qpms_irot3_t; /* qpms_types.h:296 */
+typedef struct qpms_quat_t { _Complex double a, b; } qpms_quat_t; /* qpms_types.h:261 */
+typedef struct qpms_irot3_t { qpms_quat_t rot; short det; } qpms_irot3_t; /* qpms_types.h:275 */
 
-/* qpms_types.h:306-323 */
+/* qpms_types.h:290-307 */
 typedef struct qpms_vswf_set_spec_t {
 	size_t n;
 	qpms_uvswfi_t *ilist;
@@ -119,7 +119,7 @@ typedef struct qpms_vswf_set_spec_t {
 	qpms_normalisation_t norm;
 } qpms_vswf_set_spec_t;
 
-/* qpms_types.h:346-356 */
+/* qpms_types.h:338-349 */
 typedef struct qpms_tmatrix_t {
 	const qpms_vswf_set_spec_t *spec;
 	_Complex double *m;

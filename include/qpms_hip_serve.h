@@ -82,7 +82,8 @@ typedef struct qpms_hip_response_t {
 } qpms_hip_response_t;
 
 typedef struct qpms_hip_server_opts_t {
-	size_t arena_bytes;         /* device arena for resident requests (0: 64 GiB) */
+	size_t arena_bytes;         /* device arena for resident requests (0: the device's free memory at
+	                               creation less max(8 GiB, 1/16 of the device), rounded down to GiB) */
 	int nstreams;               /* concurrent streams, at most 8 (0: 8) */
 	int mixed;                  /* 1: FP32 LU + FP64 refinement for solves; 0: FP64 LU */
 	size_t batch_max_dim;       /* plane-wave solves with L <= this are batched (0: 8192) */
@@ -92,8 +93,15 @@ typedef struct qpms_hip_server_opts_t {
 
 typedef struct qpms_hip_server_t qpms_hip_server_t;
 
+/* Returns NULL (after a message on stderr) when there is no device or the
+ * arena or the pinned staging pool cannot be allocated; nothing is left
+ * allocated then. */
 qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts);
+/* Waits for the work in flight, then releases every device resource the
+ * server holds: arena, staging pool, lanes and their per-stream LU state. */
 void qpms_hip_server_destroy(qpms_hip_server_t *srv);
+/* Size of the server's device arena in bytes. */
+size_t qpms_hip_server_arena_bytes(const qpms_hip_server_t *srv);
 /* Seconds since the server was created (the clock of t_submit/t_start/t_done). */
 double qpms_hip_server_clock(const qpms_hip_server_t *srv);
 /* Queue a request (host data copied); returns its id (> 0) or 0 if it can

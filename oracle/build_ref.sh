@@ -2,7 +2,7 @@
 # Compiles the reference's GSL-free sources on the hot path (Gaunt
 # coefficients by Xu's recursion; 2D lattice helpers and point generators)
 # straight from /root/reference into oracle/_ref/libqpms_ref.so. The result
-# is test infrastructure: it pins the oracle (tests/test_oracle_ref.py) and is
+# is test infrastructure: it pins the oracle (tests/test_oracle_pins.py) and is
 # never linked into the product. The rest of the reference path needs GSL,
 # LAPACKE/CBLAS and Amos, which this image lacks, so it is not built.
 set -e

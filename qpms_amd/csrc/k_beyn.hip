@@ -17,6 +17,7 @@
 #include <cstring>
 #include "dev_complex.hpp"
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(64) void k_small_eig(int K, const v2d *__restrict__
 
 extern "C" int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
                              double *A0, double *A1, double *A0c, double *A1c, void *stream) {
-  hipLaunchKernelGGL(k_beyn_accum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+  QH_LAUNCH(k_beyn_accum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (const v2d *)X, dzr, dzi, wr, wi, coarse, (v2d *)A0, (v2d *)A1, (v2d *)A0c, (v2d *)A1c);
   HIPCHK(hipGetLastError());
   return 0;
@@ -374,7 +375,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
   v2d *Vt = (v2d *)take((size_t)l * l * sizeof(v2d));
   unsigned long long *doff = (unsigned long long *)take(256);
   double *dn = (double *)take((size_t)l * sizeof(double));
-  hipLaunchKernelGGL(k_transpose, dim3((unsigned)(((long)m * l + 255) / 256)), dim3(256), 0, s, m, l, (const v2d *)A, Ut);
+  QH_LAUNCH(k_transpose, dim3((unsigned)(((long)m * l + 255) / 256)), dim3(256), 0, s, m, l, (const v2d *)A, Ut);
   v2d *I = (v2d *)calloc((size_t)l * l, sizeof(v2d));
   for (int i = 0; i < l; ++i) I[i * l + i] = v2d{1.0, 0.0};
   HIPCHK(hipMemcpyAsync(Vt, I, (size_t)l * l * sizeof(v2d), hipMemcpyHostToDevice, s));
@@ -400,7 +401,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
   for (int sweep = 0; sweep < 40; ++sweep) {
     HIPCHK(hipMemsetAsync(doff, 0, sizeof(unsigned long long), s));
     for (int r = 0; r < n - 1; ++r)
-      hipLaunchKernelGGL(k_jacobi_round, dim3(npr), dim3(256), 0, s, m, l, Ut, Vt, dpr + (size_t)r * npr * 2, doff);
+      QH_LAUNCH(k_jacobi_round, dim3(npr), dim3(256), 0, s, m, l, Ut, Vt, dpr + (size_t)r * npr * 2, doff);
     unsigned long long off = 0;
     HIPCHK(hipMemcpyAsync(&off, doff, sizeof off, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -409,7 +410,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
     if (offd < 1e-15) break;
   }
   if (!ws) qh_dfree(dpr);
-  hipLaunchKernelGGL(k_row_norms, dim3(l), dim3(256), 0, s, m, (const v2d *)Ut, dn);
+  QH_LAUNCH(k_row_norms, dim3(l), dim3(256), 0, s, m, (const v2d *)Ut, dn);
   double *nrm = (double *)malloc(l * sizeof(double));
   HIPCHK(hipMemcpyAsync(nrm, dn, l * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -445,7 +446,7 @@ extern "C" int qh_jacobi_svd(int m, int l, const double *A, double *Ut_out, doub
 
 extern "C" int qh_ctrans_gemm(int m, int l, int K, const double *Ut, const double *A, double *C, void *stream) {
   if (K <= 0) return 0;
-  hipLaunchKernelGGL(k_ctrans_gemm, dim3(K), dim3(l), 0, (hipStream_t)qh_stream(stream), m, l, K, (const v2d *)Ut,
+  QH_LAUNCH(k_ctrans_gemm, dim3(K), dim3(l), 0, (hipStream_t)qh_stream(stream), m, l, K, (const v2d *)Ut,
                      (const v2d *)A, (v2d *)C);
   HIPCHK(hipGetLastError());
   return 0;
@@ -453,7 +454,7 @@ extern "C" int qh_ctrans_gemm(int m, int l, int K, const double *Ut, const doubl
 
 extern "C" int qh_v0s(int m, int K, const double *S, const double *Ut, double *Y, void *stream) {
   if (K <= 0) return 0;
-  hipLaunchKernelGGL(k_v0s, dim3((unsigned)(((long)K * m + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream),
+  QH_LAUNCH(k_v0s, dim3((unsigned)(((long)K * m + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream),
                      m, K, (const v2d *)S, (const v2d *)Ut, (v2d *)Y);
   HIPCHK(hipGetLastError());
   return 0;
@@ -461,7 +462,7 @@ extern "C" int qh_v0s(int m, int K, const double *S, const double *Ut, double *Y
 
 extern "C" int qh_bmat(int K, int l, const double *T, const double *Vt, const double *sigma, double *B,
                        void *stream) {
-  hipLaunchKernelGGL(k_bmat, dim3(K), dim3(64), 0, (hipStream_t)qh_stream(stream), K, l, (const v2d *)T,
+  QH_LAUNCH(k_bmat, dim3(K), dim3(64), 0, (hipStream_t)qh_stream(stream), K, l, (const v2d *)T,
                      (const v2d *)Vt, sigma, (v2d *)B);
   HIPCHK(hipGetLastError());
   return 0;
@@ -472,7 +473,7 @@ extern "C" int qh_small_eig(int K, const double *B, double *lam, double *S, void
   if (K > KMAX) qpms_hip_die(__FILE__, __LINE__, __func__, "reduced eigenproblem of order %d exceeds %d", K, KMAX);
   hipStream_t s = (hipStream_t)qh_stream(stream);
   int *st = ws ? (int *)ws : (int *)qh_dmalloc(sizeof(int));
-  hipLaunchKernelGGL(k_small_eig, dim3(1), dim3(64), 0, s, K, (const v2d *)B, (v2d *)lam, (v2d *)S, st);
+  QH_LAUNCH(k_small_eig, dim3(1), dim3(64), 0, s, K, (const v2d *)B, (v2d *)lam, (v2d *)S, st);
   HIPCHK(hipGetLastError());
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, st, sizeof(int), hipMemcpyDeviceToHost, s));

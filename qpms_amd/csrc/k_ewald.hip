@@ -14,6 +14,7 @@
 #include "dev_ewald.hpp"
 #include "dev_specfunc.hpp"
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -168,22 +169,13 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
   if (z.re > 740.0) return false;  // every Gamma(1/2-j, z) underflows
   cplx G[JM + 1];
   if (branch == 0 && z.im == 0.0 && z.re > 0.5) {
-    // pre_j = x^{-1/2-j} e^{-x} from one exp and one square root; the
-    // continued fraction for the lowest order only, then upward in the order,
-    // Gamma(a+1, x) = a Gamma(a, x) + x^a e^{-x} (a = 1/2 - j; for x > 1/2 the
-    // added term dominates, so no digits are lost)
-    const double xi = 1.0 / z.re;
-    double pre[JM + 1];
-    pre[0] = exp(-z.re) * sqrt(xi);
+    // evanescent order of a real wavenumber: e^{x} Gamma(1/2 - j, x) from
+    // sqrt(pi) erfcx(sqrt x) by the downward recurrence in the order, then e^{-x}
+    double E[JM + 1];
+    qew::gamma_half_scaled<JM>(z.re, c_weid, E);
+    const double ex = exp(-z.re);
 #pragma unroll
-    for (int j = 1; j <= JM; ++j) pre[j] = pre[j - 1] * xi;
-    double g = qew::gamma_inc_half_real_cf(JM, z.re) * pre[JM];
-    G[JM] = mk(g, 0.0);
-#pragma unroll
-    for (int j = JM; j >= 1; --j) {
-      g = (0.5 - j) * g + pre[j - 1];
-      G[j - 1] = mk(g, 0.0);
-    }
+    for (int j = 0; j <= JM; ++j) G[j] = mk(E[j] * ex, 0.0);
   } else {
 #pragma unroll
     for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
@@ -453,8 +445,8 @@ extern "C" int qh_launch_sigma_short(int LE, int xy, int ntrip, const double *tr
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
   const dim3 g(ntrip), b(64);
-#define QH_SS(L) case L: if (xy) hipLaunchKernelGGL((k_sigma_short<L, true>), g, b, 0, s, ntrip, tr_dev, npts, pts_dev, eta, \
-      leg0_dev, csphase, (v2d *)out_dev, err_dev); else hipLaunchKernelGGL((k_sigma_short<L, false>), g, b, 0, s, ntrip, tr_dev, \
+#define QH_SS(L) case L: if (xy) QH_LAUNCH((k_sigma_short<L, true>), g, b, 0, s, ntrip, tr_dev, npts, pts_dev, eta, \
+      leg0_dev, csphase, (v2d *)out_dev, err_dev); else QH_LAUNCH((k_sigma_short<L, false>), g, b, 0, s, ntrip, tr_dev, \
       npts, pts_dev, eta, leg0_dev, csphase, (v2d *)out_dev, err_dev); break;
   switch (LE) {
     QH_SS(3) QH_SS(5) QH_SS(7) QH_SS(9)
@@ -470,8 +462,9 @@ extern "C" int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int
                                     const int *s1off_dev, double *out_dev, double *err_dev, void *stream) {
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
+  ensure_weid();   // the evanescent orders use the erfcx expansion
   const dim3 g(ntrip), b(64);
-#define QH_SL(L) case L: hipLaunchKernelGGL(k_sigma_long<L>, g, b, 0, s, ntrip, tr_dev, npts, K_dev, eta, area, leg0_dev, \
+#define QH_SL(L) case L: QH_LAUNCH(k_sigma_long<L>, g, b, 0, s, ntrip, tr_dev, npts, K_dev, eta, area, leg0_dev, \
       (const v2d *)s1_dev, s1off_dev, (v2d *)out_dev, err_dev); break;
   switch (LE) {
     QH_SL(3) QH_SL(5) QH_SL(7) QH_SL(9)
@@ -490,11 +483,12 @@ extern "C" int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_d
                                            const int *s1off_dev, double *F_dev, double *out_dev, void *stream) {
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
+  ensure_weid();   // the evanescent orders use the erfcx expansion
   const dim3 gc((npts + 63) / 64), gs((ntrip + 255) / 256);
 #define QH_LS(L) case L: \
-    if (npts > 0) hipLaunchKernelGGL(k_long_coef<L>, gc, dim3(64), 0, s, tr_dev, npts, K_dev, eta, area, leg0_dev, \
+    if (npts > 0) QH_LAUNCH(k_long_coef<L>, gc, dim3(64), 0, s, tr_dev, npts, K_dev, eta, area, leg0_dev, \
                                      (const v2d *)s1_dev, s1off_dev, (v2d *)F_dev); \
-    hipLaunchKernelGGL(k_long_shifts<L>, gs, dim3(256), 0, s, ntrip, tr_dev, npts, K_dev, (const v2d *)F_dev, \
+    QH_LAUNCH(k_long_shifts<L>, gs, dim3(256), 0, s, ntrip, tr_dev, npts, K_dev, (const v2d *)F_dev, \
                        (v2d *)out_dev); break;
   switch (LE) {
     QH_LS(3) QH_LS(5) QH_LS(7) QH_LS(9)
@@ -507,7 +501,7 @@ extern "C" int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_d
 
 extern "C" int qh_launch_sigma0(int n, const double *k_dev, double eta, double leg00, double *out_dev, void *stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_sigma0, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)qh_stream(stream), n, k_dev, eta, leg00,
+  QH_LAUNCH(k_sigma0, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)qh_stream(stream), n, k_dev, eta, leg00,
                      (v2d *)out_dev);
   HIPCHK(hipGetLastError());
   return 0;
@@ -516,7 +510,7 @@ extern "C" int qh_launch_sigma0(int n, const double *k_dev, double eta, double l
 extern "C" int qh_launch_sigma_total(int ntrip, int nsc, const double *ss, const double *sl, const double *s0,
                                      const int *flag0, int parts, double *tot, void *stream) {
   if (ntrip <= 0) return 0;
-  hipLaunchKernelGGL(k_sigma_total, dim3((nsc + 63) / 64, ntrip), dim3(64), 0, (hipStream_t)qh_stream(stream), ntrip, nsc,
+  QH_LAUNCH(k_sigma_total, dim3((nsc + 63) / 64, ntrip), dim3(64), 0, (hipStream_t)qh_stream(stream), ntrip, nsc,
                      (const v2d *)ss, (const v2d *)sl, (const v2d *)s0, flag0, parts, (v2d *)tot);
   HIPCHK(hipGetLastError());
   return 0;
@@ -526,7 +520,7 @@ extern "C" int qh_launch_e32_combine(const qh_trans_tables *t, int ntrip, int ns
                                      double *A_dev, double *B_dev, void *stream) {
   if (ntrip <= 0) return 0;
   const int ne = t->nelem;
-  hipLaunchKernelGGL(k_e32_combine, dim3((ne * ne + 255) / 256, ntrip), dim3(256), 0, (hipStream_t)qh_stream(stream),
+  QH_LAUNCH(k_e32_combine, dim3((ne * ne + 255) / 256, ntrip), dim3(256), 0, (hipStream_t)qh_stream(stream),
                      ntrip, ne, t->qslots, nsc, (const v2d *)sig_dev, (const v2d *)t->dA, (const v2d *)t->dB, t->dqA,
                      t->dqB, bq_mode, (v2d *)A_dev, (v2d *)B_dev);
   HIPCHK(hipGetLastError());

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include "dev_specfunc.hpp"
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -288,10 +289,10 @@ extern "C" int qh_launch_planewave(int lMax, size_t npart, const double *pos_dev
   const dim3 grid((unsigned)((npart * ne + 255) / 256)), block(256);
   v2d *o = (v2d *)out_dev;
   switch (lMax) {
-    case 1: hipLaunchKernelGGL(k_planewave<1>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
-    case 2: hipLaunchKernelGGL(k_planewave<2>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
-    case 3: hipLaunchKernelGGL(k_planewave<3>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
-    case 4: hipLaunchKernelGGL(k_planewave<4>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 1: QH_LAUNCH(k_planewave<1>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 2: QH_LAUNCH(k_planewave<2>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 3: QH_LAUNCH(k_planewave<3>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
+    case 4: QH_LAUNCH(k_planewave<4>, grid, block, 0, s, (int)npart, pos_dev, kvec[0], kvec[1], kvec[2], E_dev, o, ld); break;
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", lMax);
   }
   HIPCHK(hipGetLastError());
@@ -303,7 +304,7 @@ extern "C" int qh_launch_apply_t(size_t npart, int nb, const int *ttype_dev, con
   if (npart == 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
   const dim3 grid((unsigned)((npart * nb + 255) / 256)), block(256);
-  hipLaunchKernelGGL(k_apply_t, grid, block, 0, s, (int)npart, nb, ttype_dev, (const v2d *)T_dev,
+  QH_LAUNCH(k_apply_t, grid, block, 0, s, (int)npart, nb, ttype_dev, (const v2d *)T_dev,
                      (const v2d *)a_dev, (v2d *)out_dev);
   HIPCHK(hipGetLastError());
   return 0;
@@ -313,7 +314,7 @@ extern "C" int qh_launch_neg_re_cdot(int batch, long n, const double *a_dev, lon
                                      double *out_dev, void *stream) {
   if (batch == 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
-  hipLaunchKernelGGL(k_neg_re_cdot, dim3(batch), dim3(256), 0, s, n, (const v2d *)a_dev, sa, (const v2d *)f_dev, sf,
+  QH_LAUNCH(k_neg_re_cdot, dim3(batch), dim3(256), 0, s, n, (const v2d *)a_dev, sa, (const v2d *)f_dev, sf,
                      out_dev);
   HIPCHK(hipGetLastError());
   return 0;
@@ -336,7 +337,7 @@ extern "C" int qh_launch_scattered_e(int lMax, size_t npts, const double *where_
   const v2d *c = (const v2d *)cvf_dev;
   v2d *o = (v2d *)out_dev;
 #define QH_SCE1(LM, W, H) \
-  hipLaunchKernelGGL((k_scattered_e<LM, W, H>), grid, block, 0, s, (int)npts, where_dev, (int)npart, pos_dev, k_re, \
+  QH_LAUNCH((k_scattered_e<LM, W, H>), grid, block, 0, s, (int)npts, where_dev, (int)npart, pos_dev, k_re, \
                      k_im, kind, c, parity, o)
 #define QH_SCE(LM)                                   \
   do {                                               \

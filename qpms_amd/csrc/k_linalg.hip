@@ -31,6 +31,7 @@
 #include <mutex>
 #include <rocblas/rocblas.h>
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v2f __attribute__((ext_vector_type(2)));
@@ -228,10 +229,10 @@ __global__ void k_gather_rows(int n, int ncols, const int *perm, long sP, const 
   perm += (long)blockIdx.z * sP;
   X += (long)blockIdx.z * sX;
   Y += (long)blockIdx.z * sY;
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)n * ncols) return;
-  const int i = (int)(e / ncols), c = (int)(e % ncols);
-  Y[(long)i * ldy + c] = X[(long)perm[i] * ldx + c];
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)n * ncols; e += (long)gridDim.x * 256) {
+    const int i = (int)(e / ncols), c = (int)(e % ncols);
+    Y[(long)i * ldy + c] = X[(long)perm[i] * ldx + c];
+  }
 }
 
 // Y[i][:] = X[i][:] for n rows of ncols (batched, per-matrix strides)
@@ -240,10 +241,10 @@ __global__ void k_copy_rows(int n, int ncols, const typename C2<T>::t *X, long l
                             long ldy, long sY) {
   X += (long)blockIdx.z * sX;
   Y += (long)blockIdx.z * sY;
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)n * ncols) return;
-  const int i = (int)(e / ncols), c = (int)(e % ncols);
-  Y[(long)i * ldy + c] = X[(long)i * ldx + c];
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)n * ncols; e += (long)gridDim.x * 256) {
+    const int i = (int)(e / ncols), c = (int)(e % ncols);
+    Y[(long)i * ldy + c] = X[(long)i * ldx + c];
+  }
 }
 
 // perm from LAPACK-style pivots (single thread per matrix; used only when a
@@ -299,17 +300,19 @@ __global__ __launch_bounds__(256) void k_gemv_sub_f64(int M, int K, const v2d *_
   }
 }
 
+// Element-wise conversions and copies: grid-stride loops, so that a launch
+// covers any n (a batch of Beyn nodes at order 49152 is 4.8e9 elements, more
+// work-items than one dispatch can name).
 __global__ void k_d2f(long n, const v2d *x, v2f *y) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = v2f{(float)x[i].x, (float)x[i].y};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    y[i] = v2f{(float)x[i].x, (float)x[i].y};
 }
 __global__ void k_f2d_add(long n, const v2f *d, v2d *x) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) x[i] = v2d{x[i].x + (double)d[i].x, x[i].y + (double)d[i].y};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    x[i] = v2d{x[i].x + (double)d[i].x, x[i].y + (double)d[i].y};
 }
 __global__ void k_copy2d(long n, const v2d *x, v2d *y) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = x[i];
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = x[i];
 }
 // Sum of |z|^2 into out[blockIdx.x] (partial sums, finished on the host).
 __global__ void k_sumsq(long n, const v2d *x, double *out) {
@@ -786,31 +789,99 @@ static bool use_rocblas() {
   return v == 1;
 }
 
+// Per-stream state of the LU driver: a rocBLAS handle bound to the stream
+// and the lookahead side stream with its two events. Created on a stream's
+// first use; qh_stream_release() destroys it, and callers that destroy a
+// stream (the serving loop's lanes) must call it first, so that a later
+// stream created at the same address never inherits a dead stream's state.
+struct LaCtx {
+  hipStream_t side;
+  hipEvent_t ready, panel;
+};
+constexpr int STREAM_TAB = 64;
+struct StreamState {
+  hipStream_t key;
+  rocblas_handle blas;   // nullptr until the first library GEMM on the stream
+  bool la;               // lookahead context created
+  LaCtx lc;
+};
+static std::mutex g_tab_mu;
+static StreamState g_tab[STREAM_TAB];
+static int g_ntab = 0;
+
+static StreamState *state_for(hipStream_t s) {   // caller holds g_tab_mu
+  for (int i = 0; i < g_ntab; ++i) if (g_tab[i].key == s) return &g_tab[i];
+  if (g_ntab == STREAM_TAB)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "more than %d live streams use the LU (release them)", STREAM_TAB);
+  StreamState &e = g_tab[g_ntab++];
+  e = StreamState{};
+  e.key = s;
+  return &e;
+}
+
 static rocblas_handle blas_for(hipStream_t s) {
-  static std::mutex mu;
-  static hipStream_t keys[16];
-  static rocblas_handle vals[16];
-  static int n = 0;
-  std::lock_guard<std::mutex> g(mu);
-  for (int i = 0; i < n; ++i) if (keys[i] == s) return vals[i];
-  if (n == 16) qpms_hip_die(__FILE__, __LINE__, __func__, "too many streams for the GEMM handle table");
-  rocblas_handle h;
-  if (rocblas_create_handle(&h) != rocblas_status_success)
-    qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_create_handle failed");
-  if (rocblas_set_stream(h, s) != rocblas_status_success)
-    qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_set_stream failed");
-  keys[n] = s;
-  vals[n] = h;
-  return vals[n++];
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  StreamState *e = state_for(s);
+  if (!e->blas) {
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_create_handle failed");
+    if (rocblas_set_stream(h, s) != rocblas_status_success)
+      qpms_hip_die(__FILE__, __LINE__, __func__, "rocblas_set_stream failed");
+    e->blas = h;
+  }
+  return e->blas;
+}
+
+// Lookahead context of a caller stream: the side stream that factors the
+// next block's strips and the two events ordering it against the caller's
+// stream (default priority; created once per caller stream).
+static LaCtx *la_for(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  StreamState *e = state_for(s);
+  if (!e->la) {
+    HIPCHK(hipStreamCreateWithFlags(&e->lc.side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&e->lc.ready, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->lc.panel, hipEventDisableTiming));
+    e->la = true;
+  }
+  return &e->lc;
+}
+
+// Destroys the per-stream state of `stream` (rocBLAS handle, lookahead side
+// stream and events), after waiting for the work queued on them.
+extern "C" void qh_stream_release(void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  for (int i = 0; i < g_ntab; ++i) {
+    if (g_tab[i].key != s) continue;
+    StreamState &e = g_tab[i];
+    if (e.la) {
+      HIPCHK(hipStreamSynchronize(e.lc.side));
+      HIPCHK(hipEventDestroy(e.lc.ready));
+      HIPCHK(hipEventDestroy(e.lc.panel));
+      HIPCHK(hipStreamDestroy(e.lc.side));
+    }
+    if (e.blas) rocblas_destroy_handle(e.blas);
+    g_tab[i] = g_tab[--g_ntab];
+    return;
+  }
+}
+
+// Number of streams with live per-stream state (tests check that servers
+// release theirs).
+extern "C" int qh_stream_state_count(void) {
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  return g_ntab;
 }
 
 template <class T>
 static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::t *A, long lda, long sA,
                           const typename C2<T>::t *B, long ldb, long sB, typename C2<T>::t *C, long ldc, long sC,
-                          hipStream_t s) {
+                          hipStream_t s, bool own_only = false) {
   if (M <= 0 || N <= 0 || K <= 0) return;
-  if (!use_rocblas() || (long)M * N < 512L * 512) {
-    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0, s, M, N, K, A,
+  if (own_only || !use_rocblas() || (long)M * N < 512L * 512) {
+    QH_LAUNCH(k_gemm_sub<T>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0, s, M, N, K, A,
                        lda, sA, B, ldb, sB, C, ldc, sC);
     return;
   }
@@ -836,27 +907,6 @@ static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::
 constexpr int LU_NB = 256;   // outer block: K of the trailing GEMM
 static_assert(LU_NB / PW_ == TS_SLOTS, "one output slot per strip of an outer block");
 
-// Lookahead context of a caller stream: the side stream that factors the
-// next block's strips and the two events ordering it against the caller's
-// stream (default priority; created once per caller stream).
-struct LaCtx {
-  hipStream_t key, side;
-  hipEvent_t ready, panel;
-};
-static LaCtx *la_for(hipStream_t s) {
-  static std::mutex mu;
-  static LaCtx tab[16];
-  static int n = 0;
-  std::lock_guard<std::mutex> g(mu);
-  for (int i = 0; i < n; ++i) if (tab[i].key == s) return &tab[i];
-  if (n == 16) qpms_hip_die(__FILE__, __LINE__, __func__, "too many streams for the LU lookahead table");
-  LaCtx &c = tab[n];
-  c.key = s;
-  HIPCHK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&c.ready, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&c.panel, hipEventDisableTiming));
-  return &tab[n++];
-}
 static bool use_lookahead() {
   static int v = -1;
   if (v < 0) {
@@ -883,7 +933,8 @@ static size_t getrf_ws_bytes(int n, int batch) {
 // (single-stream callers only).
 template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
-                          int *perm, int *dinfo, hipStream_t s, char *ws = nullptr, bool lookahead = true) {
+                          int *perm, int *dinfo, hipStream_t s, char *ws = nullptr, bool lookahead = true,
+                          bool own_gemm = false) {
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
   if (n <= 0 || batch <= 0) return;
@@ -903,12 +954,11 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     ws = own;
   }
   const dim3 bts(TS_THREADS);
-  static bool lds_set = false;   // the tournament kernels stage a chunk of more than 64 KB
-  if (!lds_set) {
+  static std::once_flag lds_once;   // the tournament kernels stage a chunk of more than 64 KB
+  std::call_once(lds_once, [] {
     HIPCHK(hipFuncSetAttribute((const void *)k_ts_local<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
     HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
-    lds_set = true;
-  }
+  });
   char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
   char *fin0 = ws + 2 * (long)Gmax * setb;           // per-strip slots of the last level's output
   const long finb = (long)PanelWS<T>::fin_bytes();
@@ -921,26 +971,26 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
       const int w = min(PW, nb - j0), c0 = k0 + j0;
       char *fin = fin0 + (long)(j0 / PW) * finb;
       const int G = (n - c0 + TsRows<T>::R - 1) / TsRows<T>::R;
-      hipLaunchKernelGGL(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), st, n, c0, w, A, lda, sA, ws, sW, setb,
+      QH_LAUNCH(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), st, n, c0, w, A, lda, sA, ws, sW, setb,
                          (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
       char *cur = bufA, *nxt = bufB;
       for (int nsets = G; nsets > 1; nsets = (nsets + F - 1) / F) {
         const int nout = (nsets + F - 1) / F;
-        hipLaunchKernelGGL(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), st, c0, w, nsets, cur, nxt, sW,
+        QH_LAUNCH(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), st, c0, w, nsets, cur, nxt, sW,
                            setb, fin, nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
         char *tmp = cur; cur = nxt; nxt = tmp;
       }
-      hipLaunchKernelGGL(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, st, n, c0, w, A, lda,
+      QH_LAUNCH(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, st, n, c0, w, A, lda,
                          sA, fin, sW, k0, k0 + nb);
       if (n - c0 - w > 0)
-        hipLaunchKernelGGL(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, st, n, c0, w, A, lda, sA);
+        QH_LAUNCH(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, st, n, c0, w, A, lda, sA);
       const int rest = k0 + nb - (c0 + w);
       if (rest > 0) {
-        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, st, w, rest,
+        QH_LAUNCH((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, st, w, rest,
                            A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);
         const int mr = n - c0 - w;
         if (mr > 0)
-          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, st, mr,
+          QH_LAUNCH(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, st, mr,
                              rest, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA,
                              A + (long)(c0 + w) * lda + c0 + w, lda, sA);
       }
@@ -953,7 +1003,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   const bool la = lookahead && use_lookahead() && n > 2 * LU_NB;
   LaCtx *cx = la ? la_for(s) : nullptr;
   hipStream_t sp = la ? cx->side : s;
-  if (perm) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
+  if (perm) QH_LAUNCH(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
   if (la) {
     HIPCHK(hipEventRecord(cx->ready, s));
     HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
@@ -964,25 +1014,25 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     const int nb = min(LU_NB, n - k0);
     if (la) HIPCHK(hipStreamWaitEvent(s, cx->panel, 0));
     if (n > nb)
-      hipLaunchKernelGGL(k_ts_apply_defer<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, k0,
+      QH_LAUNCH(k_ts_apply_defer<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, k0,
                          (nb + PW - 1) / PW, A, lda, sA, fin0, finb, sW, k0, k0 + nb);
     const int nr = n - k0 - nb;
     if (nr > 0) {
       // U12 = L11^{-1} A12, PW rows at a time
       for (int j0 = 0; j0 < nb; j0 += PW) {
         const int w = min(PW, nb - j0), c0 = k0 + j0;
-        hipLaunchKernelGGL((k_trsm<T, true, PW>), dim3((nr + 255) / 256, 1, batch), b256, 0, s, w, nr,
+        QH_LAUNCH((k_trsm<T, true, PW>), dim3((nr + 255) / 256, 1, batch), b256, 0, s, w, nr,
                            A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA);
         const int below = nb - j0 - w;
         if (below > 0)
-          hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nr + GT - 1) / GT, (below + GT - 1) / GT, batch), b256, 0, s, below,
+          QH_LAUNCH(k_gemm_sub<T>, dim3((nr + GT - 1) / GT, (below + GT - 1) / GT, batch), b256, 0, s, below,
                              nr, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA,
                              A + (long)(c0 + w) * lda + k0 + nb, lda, sA);
       }
       // A22 -= L21 U12: the next block's columns first, then its strips, then the rest
       const int nb1 = min(LU_NB, nr);
       trailing_gemm<T>(nr, nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA, A + (long)k0 * lda + k0 + nb, lda,
-                       sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s);
+                       sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s, own_gemm);
       if (la) {
         HIPCHK(hipEventRecord(cx->ready, s));
         HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
@@ -992,7 +1042,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
       if (nr > nb1)
         trailing_gemm<T>(nr, nr - nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA,
                          A + (long)k0 * lda + k0 + nb + nb1, lda, sA, A + (long)(k0 + nb) * lda + k0 + nb + nb1, lda,
-                         sA, s);
+                         sA, s, own_gemm);
     }
   }
   HIPCHK(hipGetLastError());
@@ -1115,36 +1165,36 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
   const dim3 b256(256);
   const long tot = (long)n * nrhs;
   // work <- P B, then forward / backward substitution in place on work, copy back.
-  hipLaunchKernelGGL(k_gather_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs, perm, sP,
+  QH_LAUNCH(k_gather_rows<T>, dim3(qh_stride_blocks(tot), 1, batch), b256, 0, s, n, nrhs, perm, sP,
                      (const cT *)B, ldb, sB, work, (long)nrhs, sW);
   if (nrhs <= TRV_R) {
     const int nblk = (n + TRV_W - 1) / TRV_W;
     for (int bi = 0; bi < nblk; ++bi) {
       const int r0 = bi * TRV_W, w = min(TRV_W, n - r0), rem = n - r0 - w;
-      hipLaunchKernelGGL((k_trsv_wave<T, true>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      QH_LAUNCH((k_trsv_wave<T, true>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
       if (rem > 0)
-        hipLaunchKernelGGL(k_trsv_update<T>, dim3((rem + 63) / 64, 1, batch), dim3(64), 0, s, r0 + w, rem, r0, w, nrhs,
+        QH_LAUNCH(k_trsv_update<T>, dim3((rem + 63) / 64, 1, batch), dim3(64), 0, s, r0 + w, rem, r0, w, nrhs,
                            LU, lda, sA, work, sW);
     }
     for (int bi = nblk - 1; bi >= 0; --bi) {
       const int r0 = bi * TRV_W, w = min(TRV_W, n - r0);
-      hipLaunchKernelGGL((k_trsv_wave<T, false>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
+      QH_LAUNCH((k_trsv_wave<T, false>), dim3(1, 1, batch), dim3(TRV_W), 0, s, r0, w, nrhs, LU, lda, sA, work, sW);
       if (r0 > 0)
-        hipLaunchKernelGGL(k_trsv_update<T>, dim3((r0 + 63) / 64, 1, batch), dim3(64), 0, s, 0, r0, r0, w, nrhs, LU, lda,
+        QH_LAUNCH(k_trsv_update<T>, dim3((r0 + 63) / 64, 1, batch), dim3(64), 0, s, 0, r0, r0, w, nrhs, LU, lda,
                            sA, work, sW);
     }
-    hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs,
+    QH_LAUNCH(k_copy_rows<T>, dim3(qh_stride_blocks(tot), 1, batch), b256, 0, s, n, nrhs,
                        (const cT *)work, (long)nrhs, sW, B, ldb, sB);
     HIPCHK(hipGetLastError());
     return;
   }
   for (int r0 = 0; r0 < n; r0 += NBW) {
     const int w = (n - r0 < NBW) ? n - r0 : NBW;
-    hipLaunchKernelGGL((k_trsm<T, true>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
+    QH_LAUNCH((k_trsm<T, true>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
                        LU + (long)r0 * lda + r0, lda, sA, work + (long)r0 * nrhs, (long)nrhs, sW);
     const int rem = n - r0 - w;
     if (rem > 0)
-      hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (rem + GT - 1) / GT, batch), b256, 0, s, rem, nrhs, w,
+      QH_LAUNCH(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (rem + GT - 1) / GT, batch), b256, 0, s, rem, nrhs, w,
                          LU + (long)(r0 + w) * lda + r0, lda, sA, (const cT *)(work + (long)r0 * nrhs), (long)nrhs, sW,
                          work + (long)(r0 + w) * nrhs, (long)nrhs, sW);
   }
@@ -1152,14 +1202,14 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int r0 = bi * NBW;
     const int w = (n - r0 < NBW) ? n - r0 : NBW;
-    hipLaunchKernelGGL((k_trsm<T, false>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
+    QH_LAUNCH((k_trsm<T, false>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
                        LU + (long)r0 * lda + r0, lda, sA, work + (long)r0 * nrhs, (long)nrhs, sW);
     if (r0 > 0)
-      hipLaunchKernelGGL(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (r0 + GT - 1) / GT, batch), b256, 0, s, r0, nrhs, w,
+      QH_LAUNCH(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (r0 + GT - 1) / GT, batch), b256, 0, s, r0, nrhs, w,
                          LU + r0, lda, sA, (const cT *)(work + (long)r0 * nrhs), (long)nrhs, sW, work, (long)nrhs, sW);
   }
   // copy back (B may have a different leading dimension), one launch for the batch
-  hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)((tot + 255) / 256), 1, batch), b256, 0, s, n, nrhs,
+  QH_LAUNCH(k_copy_rows<T>, dim3(qh_stride_blocks(tot), 1, batch), b256, 0, s, n, nrhs,
                      (const cT *)work, (long)nrhs, sW, B, ldb, sB);
   HIPCHK(hipGetLastError());
 }
@@ -1181,7 +1231,7 @@ extern "C" int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double 
   hipStream_t s = (hipStream_t)qh_stream(stream);
   int *perm = (int *)qh_dmalloc(n * sizeof(int));
   v2d *work = (v2d *)qh_dmalloc(n * nrhs * sizeof(v2d));
-  hipLaunchKernelGGL(k_perm_from_ipiv, dim3(1), dim3(64), 0, s, (int)n, ipiv_dev, perm);
+  QH_LAUNCH(k_perm_from_ipiv, dim3(1), dim3(64), 0, s, (int)n, ipiv_dev, perm);
   getrs_batched<double>((int)n, (int)nrhs, 1, (const v2d *)LU_dev, (long)n, 0, perm, 0, (v2d *)B_dev, (long)nrhs, 0,
                         work, 0, s);
   HIPCHK(hipStreamSynchronize(s));
@@ -1192,7 +1242,7 @@ extern "C" int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double 
 
 extern "C" int qh_launch_block_tw(size_t nr, size_t L, const double *T, const double *W, double *M, size_t col0,
                                   void *stream) {
-  hipLaunchKernelGGL(k_block_tw, dim3((unsigned)((L + 255) / 256), (unsigned)nr), dim3(256), 0,
+  QH_LAUNCH(k_block_tw, dim3((unsigned)((L + 255) / 256), (unsigned)nr), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), (int)nr, (int)L, (const v2d *)T, (const v2d *)W, (v2d *)M,
                      (int)col0);
   HIPCHK(hipGetLastError());
@@ -1248,14 +1298,14 @@ extern "C" int qh_gemm_sub_f64(int M, int N, int K, int batch, const double *A, 
   if (N <= 4) {   // few right-hand sides: stream A once, a wave per row
     const dim3 g((unsigned)((M + 3) / 4), (unsigned)batch);
     hipStream_t s = (hipStream_t)qh_stream(stream);
-#define QH_GV(NR) hipLaunchKernelGGL(k_gemv_sub_f64<NR>, g, dim3(256), 0, s, M, K, (const v2d *)A, lda, sA, \
+#define QH_GV(NR) QH_LAUNCH(k_gemv_sub_f64<NR>, g, dim3(256), 0, s, M, K, (const v2d *)A, lda, sA, \
                                      (const v2d *)B, ldb, sB, (v2d *)C, ldc, sC)
     switch (N) { case 1: QH_GV(1); break; case 2: QH_GV(2); break; case 3: QH_GV(3); break; default: QH_GV(4); }
 #undef QH_GV
     HIPCHK(hipGetLastError());
     return 0;
   }
-  hipLaunchKernelGGL(k_gemm_sub<double>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
+  QH_LAUNCH(k_gemm_sub<double>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), M, N, K, (const v2d *)A, lda, sA, (const v2d *)B, ldb, sB,
                      (v2d *)C, ldc, sC);
   HIPCHK(hipGetLastError());
@@ -1263,24 +1313,24 @@ extern "C" int qh_gemm_sub_f64(int M, int N, int K, int batch, const double *A, 
 }
 extern "C" int qh_gemm_sub_f32(int M, int N, int K, int batch, const float *A, long lda, long sA, const float *B,
                                long ldb, long sB, float *C, long ldc, long sC, void *stream) {
-  hipLaunchKernelGGL(k_gemm_sub<float>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
+  QH_LAUNCH(k_gemm_sub<float>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), M, N, K, (const v2f *)A, lda, sA, (const v2f *)B, ldb, sB,
                      (v2f *)C, ldc, sC);
   HIPCHK(hipGetLastError());
   return 0;
 }
 extern "C" int qh_d2f(long n, const double *x, float *y, void *stream) {
-  hipLaunchKernelGGL(k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+  QH_LAUNCH(k_d2f, dim3(qh_stride_blocks(n)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (const v2d *)x, (v2f *)y);
   return 0;
 }
 extern "C" int qh_f2d_add(long n, const float *d, double *x, void *stream) {
-  hipLaunchKernelGGL(k_f2d_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+  QH_LAUNCH(k_f2d_add, dim3(qh_stride_blocks(n)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (const v2f *)d, (v2d *)x);
   return 0;
 }
 extern "C" int qh_copy_c64(long n, const double *x, double *y, void *stream) {
-  hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+  QH_LAUNCH(k_copy2d, dim3(qh_stride_blocks(n)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (const v2d *)x, (v2d *)y);
   return 0;
 }
@@ -1289,7 +1339,7 @@ extern "C" double qh_norm2_c64(long n, const double *x, void *stream) {
   hipStream_t s = (hipStream_t)qh_stream(stream);
   const int nb = 512;
   double *part = (double *)qh_dmalloc(nb * sizeof(double));
-  hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, s, n, (const v2d *)x, part);
+  QH_LAUNCH(k_sumsq, dim3(nb), dim3(256), 0, s, n, (const v2d *)x, part);
   double h[nb];
   HIPCHK(hipMemcpyAsync(h, part, sizeof h, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -12,6 +12,7 @@
 //           work is dead. 16 flops per lane per iteration.
 #include <hip/hip_runtime.h>
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -61,7 +62,7 @@ extern "C" int qpms_hip_probe_triad(size_t n, double *a, const double *b, const 
   if (n & 1) qpms_hip_die(__FILE__, __LINE__, __func__, "triad length %zu must be even", n);
   hipStream_t st = (hipStream_t)qh_stream(stream);
   const long n2 = (long)(n / 2);
-  hipLaunchKernelGGL(k_triad, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a,
+  QH_LAUNCH(k_triad, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a,
                      (const v2d *)b, (const v2d *)c, s);
   HIPCHK(hipGetLastError());
   return 0;
@@ -73,7 +74,7 @@ extern "C" int qpms_hip_probe_fill(size_t n, double *a, double s, void *stream) 
   if (n & 1) qpms_hip_die(__FILE__, __LINE__, __func__, "fill length %zu must be even", n);
   hipStream_t st = (hipStream_t)qh_stream(stream);
   const long n2 = (long)(n / 2);
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a, s);
+  QH_LAUNCH(k_fill, dim3((unsigned)((n2 + PROBE_CHUNK - 1) / PROBE_CHUNK)), dim3(256), 0, st, n2, (v2d *)a, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -83,7 +84,7 @@ extern "C" int qpms_hip_probe_fill(size_t n, double *a, double s, void *stream) 
 extern "C" int qpms_hip_probe_fma64(int blocks, int iters, double *out, void *stream) {
   if (blocks <= 0 || iters <= 0) return 0;
   hipStream_t st = (hipStream_t)qh_stream(stream);
-  hipLaunchKernelGGL(k_fma64, dim3(blocks), dim3(256), 0, st, iters, out);
+  QH_LAUNCH(k_fma64, dim3(blocks), dim3(256), 0, st, iters, out);
   HIPCHK(hipGetLastError());
   return 0;
 }

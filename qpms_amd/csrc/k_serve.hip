@@ -6,6 +6,7 @@
 #include <cstdint>
 #include "dev_complex.hpp"
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -202,6 +203,32 @@ __global__ void k_pair_sigma_total(int np, int nsc, const v2d *__restrict__ ss, 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
   qpms_hip_die(__FILE__, __LINE__, __func__, "HIP error %d (%s) in %s", (int)e_, hipGetErrorString(e_), #x); } while (0)
 
+// Clears `bytes` (a multiple of 4) at a 16-byte-aligned address: 16-byte
+// stores over the bulk, then the 4-byte words of the tail. A kernel rather
+// than hipMemsetAsync, so the serving loop's launch sequences stay plain
+// kernel nodes when a caller captures them in a graph.
+__global__ __launch_bounds__(256) void k_zero(uint4 *__restrict__ p, long n16, unsigned *__restrict__ tail, int ntail) {
+  const long stride = (long)gridDim.x * 256;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) p[i] = uint4{0, 0, 0, 0};
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0u;
+}
+
+extern "C" int qh_zero(void *dst, size_t bytes, void *stream) {
+  if (bytes == 0) return 0;
+  if (((uintptr_t)dst & 15) || (bytes & 3))
+    qpms_hip_die(__FILE__, __LINE__, __func__, "qh_zero: %zu bytes at %p (needs 16-byte alignment, 4-byte multiple)",
+                 bytes, dst);
+  const long n16 = (long)(bytes / 16);
+  const int ntail = (int)((bytes % 16) / 4);
+  long blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  QH_LAUNCH(k_zero, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)qh_stream(stream), (uint4 *)dst, n16,
+                     (unsigned *)((char *)dst + n16 * 16), ntail);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" void qh_probe_host(long n, unsigned long long seed, double *V) {
   for (long i = 0; i < n; ++i) {
     const v2d v = qh_probe_value(seed, (uint64_t)i);
@@ -212,7 +239,7 @@ extern "C" void qh_probe_host(long n, unsigned long long seed, double *V) {
 
 extern "C" int qh_launch_probe(long n, unsigned long long seed, double *V, void *stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+  QH_LAUNCH(k_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (uint64_t)seed, (v2d *)V);
   HIPCHK(hipGetLastError());
   return 0;
@@ -220,7 +247,7 @@ extern "C" int qh_launch_probe(long n, unsigned long long seed, double *V, void 
 
 extern "C" int qh_launch_expand_tdiag(int lMax, int nsets, const double *R, double *td, void *stream) {
   const int nb = 2 * lMax * (lMax + 2);
-  hipLaunchKernelGGL(k_expand_tdiag, dim3((nsets * nb + 255) / 256), dim3(256), 0, (hipStream_t)qh_stream(stream),
+  QH_LAUNCH(k_expand_tdiag, dim3((nsets * nb + 255) / 256), dim3(256), 0, (hipStream_t)qh_stream(stream),
                      lMax, nsets, (const v2d *)R, (v2d *)td);
   HIPCHK(hipGetLastError());
   return 0;
@@ -228,7 +255,7 @@ extern "C" int qh_launch_expand_tdiag(int lMax, int nsets, const double *R, doub
 
 extern "C" int qh_launch_tdiag_mul(int batch, int L, int nb, int npad, const double *td, long std_, const double *a,
                                    long sa, double *out, long so, void *stream) {
-  hipLaunchKernelGGL(k_tdiag_mul, dim3((npad + 255) / 256, batch), dim3(256), 0, (hipStream_t)qh_stream(stream), L, nb,
+  QH_LAUNCH(k_tdiag_mul, dim3((npad + 255) / 256, batch), dim3(256), 0, (hipStream_t)qh_stream(stream), L, nb,
                      npad, (const v2d *)td, std_, (const v2d *)a, sa, (v2d *)out, so);
   HIPCHK(hipGetLastError());
   return 0;
@@ -237,7 +264,7 @@ extern "C" int qh_launch_tdiag_mul(int batch, int L, int nb, int npad, const dou
 extern "C" int qh_launch_pad_identity(int batch, int L, int npad, double *M, long sM, void *stream) {
   if (L >= npad) return 0;
   const long tot = (long)npad * npad;
-  hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((tot + 255) / 256), 1, batch), dim3(256), 0,
+  QH_LAUNCH(k_pad_identity, dim3((unsigned)((tot + 255) / 256), 1, batch), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), L, npad, (v2d *)M, sM);
   HIPCHK(hipGetLastError());
   return 0;
@@ -250,14 +277,14 @@ extern "C" int qh_launch_sumsq_batched(int batch, long n, const double *x, long 
   const long p = (n + 4095) / 4096;
   const int P = (int)(p < 1 ? 1 : (p > QH_SUMSQ_PARTS ? QH_SUMSQ_PARTS : p));
   hipStream_t s = (hipStream_t)qh_stream(stream);
-  hipLaunchKernelGGL(k_sumsq_part, dim3(P, batch), dim3(256), 0, s, n, (const v2d *)x, sx, P, part);
-  hipLaunchKernelGGL(k_sumsq_fin, dim3(batch), dim3(256), 0, s, P, (const double *)part, out);
+  QH_LAUNCH(k_sumsq_part, dim3(P, batch), dim3(256), 0, s, n, (const v2d *)x, sx, P, part);
+  QH_LAUNCH(k_sumsq_fin, dim3(batch), dim3(256), 0, s, P, (const double *)part, out);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 extern "C" int qh_launch_logdet(int n, const double *LU, const int *ipiv, double *out, void *stream) {
-  hipLaunchKernelGGL(k_logdet, dim3(1), dim3(256), 0, (hipStream_t)qh_stream(stream), n, (const v2d *)LU, ipiv, out);
+  QH_LAUNCH(k_logdet, dim3(1), dim3(256), 0, (hipStream_t)qh_stream(stream), n, (const v2d *)LU, ipiv, out);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -268,7 +295,7 @@ extern "C" int qh_launch_e32_assemble_M(const qh_trans_tables *t, int np, int ns
   const long npairs = (long)np * np;
   for (long t0 = 0; t0 < npairs; t0 += 65535) {
     const int cnt = (int)((npairs - t0 < 65535) ? npairs - t0 : 65535);
-    hipLaunchKernelGGL(k_e32_assemble_M, dim3((nb * nb + 255) / 256, cnt), dim3(256), 0,
+    QH_LAUNCH(k_e32_assemble_M, dim3((nb * nb + 255) / 256, cnt), dim3(256), 0,
                        (hipStream_t)qh_stream(stream), np, t->nelem, t->qslots, nsc, (const v2d *)sig,
                        (const v2d *)t->dA, (const v2d *)t->dB, t->dqA, t->dqB, bq_mode, ttype, (const v2d *)tdiag,
                        (v2d *)M, ld, (int)t0);
@@ -280,7 +307,7 @@ extern "C" int qh_launch_e32_assemble_M(const qh_trans_tables *t, int np, int ns
 extern "C" int qh_launch_pair_triples(int np, const double *pos, double bx, double by, double kre, double kim,
                                       double *tr, void *stream) {
   const long n = (long)np * np;
-  hipLaunchKernelGGL(k_pair_triples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream),
+  QH_LAUNCH(k_pair_triples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream),
                      np, pos, bx, by, kre, kim, tr);
   HIPCHK(hipGetLastError());
   return 0;
@@ -289,7 +316,7 @@ extern "C" int qh_launch_pair_triples(int np, const double *pos, double bx, doub
 extern "C" int qh_launch_pair_sigma_total(int np, int nsc, const double *ss, const double *sl, const double *s0,
                                           double *tot, void *stream) {
   const long n = (long)np * np * nsc;
-  hipLaunchKernelGGL(k_pair_sigma_total, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  QH_LAUNCH(k_pair_sigma_total, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)qh_stream(stream), np, nsc, (const v2d *)ss, (const v2d *)sl, (const v2d *)s0,
                      (v2d *)tot);
   HIPCHK(hipGetLastError());

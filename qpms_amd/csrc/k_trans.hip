@@ -23,6 +23,7 @@
 #include <utility>
 #include "dev_specfunc.hpp"
 #include "qpms_internal.h"
+#include "dev_launch.hpp"
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -281,10 +282,10 @@ extern "C" int qh_launch_ab_batch(const qh_trans_tables *t, size_t npts, const d
   const v2d *kr = (const v2d *)kr_dev;
   v2d *A = (v2d *)A_dev, *B = (v2d *)B_dev;
   switch (t->lMax) {
-    case 1: hipLaunchKernelGGL(k_ab_batch<1>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
-    case 2: hipLaunchKernelGGL(k_ab_batch<2>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
-    case 3: hipLaunchKernelGGL(k_ab_batch<3>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
-    case 4: hipLaunchKernelGGL(k_ab_batch<4>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
+    case 1: QH_LAUNCH(k_ab_batch<1>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
+    case 2: QH_LAUNCH(k_ab_batch<2>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
+    case 3: QH_LAUNCH(k_ab_batch<3>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
+    case 4: QH_LAUNCH(k_ab_batch<4>, grid, block, 0, s, (int)npts, kr, theta_dev, phi_dev, kind, mA, mB, A, B); break;
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", t->lMax);
   }
   HIPCHK(hipGetLastError());
@@ -304,7 +305,7 @@ static void launch_w(const qh_trans_tables *t, size_t npart, const double *pos_d
   const v2d *td = (const v2d *)tdiag;
   v2d *W = (v2d *)W_dev;
   switch (t->lMax) {
-#define QH_W_CASE(LM) case LM: hipLaunchKernelGGL((k_w_build<LM, MODE_M>), grid, block, 0, s, pos_dev, (int)npart, \
+#define QH_W_CASE(LM) case LM: QH_LAUNCH((k_w_build<LM, MODE_M>), grid, block, 0, s, pos_dev, (int)npart, \
       (int)row_p0, nchunks, k_re, k_im, kind, mA, mB, ttype, td, W, ld); break;
     QH_W_CASE(1) QH_W_CASE(2) QH_W_CASE(3) QH_W_CASE(4)
 #undef QH_W_CASE
@@ -335,7 +336,7 @@ extern "C" int qh_launch_modeproblem(const qh_trans_tables *t, size_t npart, con
 extern "C" int qh_launch_mie(int lMax, int nsph, const double *params_dev, double *out_dev, void *stream) {
   if (nsph <= 0) return 0;
   if (lMax + 1 > MIE_L) qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d too large for Mie kernel", lMax);
-  hipLaunchKernelGGL(k_mie, dim3((nsph + 63) / 64), dim3(64), 0, (hipStream_t)qh_stream(stream), lMax, nsph,
+  QH_LAUNCH(k_mie, dim3((nsph + 63) / 64), dim3(64), 0, (hipStream_t)qh_stream(stream), lMax, nsph,
                      params_dev, (v2d *)out_dev);
   HIPCHK(hipGetLastError());
   return 0;

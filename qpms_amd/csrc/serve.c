@@ -45,6 +45,7 @@ int qh_launch_sumsq_batched(int batch, long n, const double *x, long sx, double 
 #define QH_SUMSQ_PARTS 512
 int qh_launch_logdet(int n, const double *LU, const int *ipiv, double *out, void *stream);
 int qh_launch_probe(long n, unsigned long long seed, double *V, void *stream);
+void qh_stream_release(void *stream);
 void qh_probe_host(long n, unsigned long long seed, double *V);
 int qh_launch_pair_triples(int np, const double *pos, double bx, double by, double kre, double kim,
 		double *tr, void *stream);
@@ -80,15 +81,22 @@ typedef struct {
 	size_t in_use;
 } qh_arena;
 
-static void arena_init(qh_arena *a, size_t size, int pinned_host) {
+/* Returns 0, or the HIP error of the allocation (nothing is left allocated). */
+static int arena_init(qh_arena *a, size_t size, int pinned_host) {
+	memset(a, 0, sizeof(*a));
+	const hipError_t e = pinned_host ? hipHostMalloc((void **)&a->base, size, 0) : hipMalloc((void **)&a->base, size);
+	if (e != hipSuccess) {
+		(void)hipGetLastError();   /* clear the sticky error of the failed allocation */
+		a->base = NULL;
+		return (int)e;
+	}
 	a->size = size;
-	if (pinned_host) SRV_HIPCHK(hipHostMalloc((void **)&a->base, size, 0));
-	else SRV_HIPCHK(hipMalloc((void **)&a->base, size));
 	a->cap = 64;
 	QH_MALLOC(a->b, a->cap * sizeof(qh_blk));
 	a->b[0] = (qh_blk){0, size, 0};
 	a->nb = 1;
 	a->in_use = 0;
+	return 0;
 }
 
 static long arena_alloc(qh_arena *a, size_t bytes) {
@@ -262,12 +270,50 @@ static const qpms_trans_calculator *srv_calc(qpms_hip_server_t *s, int lMax) {
 	return s->calc[lMax];
 }
 
+/* Device memory a server leaves to the rest of the process when it sizes its
+ * arena itself: the larger of 8 GiB and 1/16 of the device. */
+static size_t auto_headroom(size_t total) {
+	const size_t h = total / 16, floor_ = (size_t)8 << 30;
+	return h > floor_ ? h : floor_;
+}
+
+static void server_release(qpms_hip_server_t *s) {
+	for (int i = 0; i < s->o.nstreams; ++i)
+		if (s->lane[i]) {
+			SRV_HIPCHK(hipStreamSynchronize(s->lane[i]));
+			qh_stream_release(s->lane[i]);
+			SRV_HIPCHK(hipStreamDestroy(s->lane[i]));
+		}
+	if (s->arena.base) SRV_HIPCHK(hipFree(s->arena.base));
+	free(s->arena.b);
+	if (s->pinned.base) SRV_HIPCHK(hipHostFree(s->pinned.base));
+	free(s->pinned.b);
+	for (int l = 1; l <= 4; ++l) if (s->calc[l]) qpms_trans_calculator_free(s->calc[l]);
+	free(s->ready);
+	free(s->order);
+	free(s);
+}
+
 qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
-	QH_ENSURE(qpms_hip_init() == 0, "no HIP device");
+	if (qpms_hip_init() != 0) {
+		fprintf(stderr, "qpms_hip_server_create: no HIP device\n");
+		return NULL;
+	}
 	qpms_hip_server_t *s;
 	QH_CALLOC(s, 1, sizeof(*s));
 	if (opts) s->o = *opts;
-	if (!s->o.arena_bytes) s->o.arena_bytes = (size_t)64 << 30;
+	size_t dfree = 0, dtotal = 0;
+	SRV_HIPCHK(hipMemGetInfo(&dfree, &dtotal));
+	if (!s->o.arena_bytes) {
+		const size_t head = auto_headroom(dtotal);
+		if (dfree <= head + ((size_t)1 << 30)) {
+			fprintf(stderr, "qpms_hip_server_create: %zu MiB free on the device, below the %zu MiB headroom\n",
+				dfree >> 20, head >> 20);
+			free(s);
+			return NULL;
+		}
+		s->o.arena_bytes = (dfree - head) & ~(((size_t)1 << 30) - 1);
+	}
 	if (s->o.nstreams <= 0) s->o.nstreams = 8;
 	if (s->o.nstreams > MAX_LANES) s->o.nstreams = MAX_LANES;
 	if (!s->o.batch_max_dim) s->o.batch_max_dim = 8192;
@@ -275,8 +321,20 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 	if (s->o.batch_max > MAX_BATCH) s->o.batch_max = MAX_BATCH;
 	if (s->o.refine_steps <= 0) s->o.refine_steps = 3;
 	for (int i = 0; i < s->o.nstreams; ++i) SRV_HIPCHK(hipStreamCreateWithFlags(&s->lane[i], hipStreamNonBlocking));
-	arena_init(&s->arena, s->o.arena_bytes, 0);
-	arena_init(&s->pinned, (size_t)64 << 20, 1);
+	int e = arena_init(&s->arena, s->o.arena_bytes, 0);
+	if (e) {
+		fprintf(stderr, "qpms_hip_server_create: device arena of %zu MiB: HIP error %d (%s); %zu of %zu MiB free\n",
+			s->o.arena_bytes >> 20, e, hipGetErrorString((hipError_t)e), dfree >> 20, dtotal >> 20);
+		server_release(s);
+		return NULL;
+	}
+	e = arena_init(&s->pinned, (size_t)64 << 20, 1);
+	if (e) {
+		fprintf(stderr, "qpms_hip_server_create: pinned staging pool: HIP error %d (%s)\n", e,
+			hipGetErrorString((hipError_t)e));
+		server_release(s);
+		return NULL;
+	}
 	for (int l = 1; l <= 4; ++l) {
 		s->calc[l] = qpms_trans_calculator_init(l, QPMS_NORMALISATION_DEFAULT);
 		qh_tables_get(s->calc[l]);   /* device tables now, not inside the loop */
@@ -286,24 +344,19 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 	return s;
 }
 
+size_t qpms_hip_server_arena_bytes(const qpms_hip_server_t *s) { return s->arena.size; }
+
 void qpms_hip_server_destroy(qpms_hip_server_t *s) {
 	if (!s) return;
 	for (int i = 0; i < s->o.nstreams; ++i) SRV_HIPCHK(hipStreamSynchronize(s->lane[i]));
 	for (int u = 0; u < MAX_UNITS; ++u)
 		if (s->unit[u].active) {
 			SRV_HIPCHK(hipEventDestroy(s->unit[u].done));
+			free(s->unit[u].bcont);
 			for (int j = 0; j < s->unit[u].njobs; ++j) { free(s->unit[u].jobs[j]->pos); free(s->unit[u].jobs[j]); }
 		}
 	while (s->qhead) { qh_job *j = s->qhead; s->qhead = j->next; free(j->pos); free(j); }
-	for (int i = 0; i < s->o.nstreams; ++i) SRV_HIPCHK(hipStreamDestroy(s->lane[i]));
-	SRV_HIPCHK(hipFree(s->arena.base));
-	free(s->arena.b);
-	SRV_HIPCHK(hipHostFree(s->pinned.base));
-	free(s->pinned.b);
-	for (int l = 1; l <= 4; ++l) qpms_trans_calculator_free(s->calc[l]);
-	free(s->ready);
-	free(s->order);
-	free(s);
+	server_release(s);
 }
 
 static size_t job_L(const qpms_hip_request_t *r) { return r->npart * 2 * (size_t)qpms_lMax2nelem(r->lMax); }
@@ -456,9 +509,9 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 		for (int c = 0; c < 3; ++c) { E[2 * c] = creal(jb->req.E[c]); E[2 * c + 1] = cimag(jb->req.E[c]); }
 	}
 	SRV_HIPCHK(hipMemcpyAsync(din, u->pin, in_bytes, hipMemcpyHostToDevice, st));
-	SRV_HIPCHK(hipMemsetAsync(a, 0, B * npad * 16, st));
-	SRV_HIPCHK(hipMemsetAsync(info, 0, B * 4, st));
-	SRV_HIPCHK(hipMemsetAsync(x, 0, B * npad * 16, st));
+	qh_zero(a, B * npad * 16, st);
+	qh_zero(info, B * 4, st);
+	qh_zero(x, B * npad * 16, st);
 	for (int j = 0; j < B; ++j) {
 		const qh_job *jb = u->jobs[j];
 		const int lMax = jb->req.lMax;
@@ -470,7 +523,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 		qh_launch_expand_tdiag(lMax, 1, mie, td, st);
 		/* k_w_build reads ttype[p] for every particle: all zeros */
 		int *tt = bump(&bp, np * 4);
-		SRV_HIPCHK(hipMemsetAsync(tt, 0, np * 4, st));
+		qh_zero(tt, np * 4, st);
 		const double k = wavenumber_re(jb->req.eps_bg, jb->req.omega);
 		qh_launch_modeproblem(qh_tables_get(srv_calc(s, lMax)), np, dpos, tt, td, k, 0.0, M + 2 * j * nn, npad, st);
 		qh_launch_pad_identity(1, (int)L, (int)npad, M + 2 * j * nn, (long)nn, st);
@@ -497,7 +550,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 		qh_getrf_batched_ws_f64((int)npad, B, M, sM, ipiv, perm, info, ws, st);
 		qh_copy_c64((long)(B * npad), b, x, st);
 		qh_getrs_batched_f64((int)npad, 1, B, M, sM, perm, x, sV, work, st);
-		SRV_HIPCHK(hipMemsetAsync(r, 0, B * npad * 16, st));
+		qh_zero(r, B * npad * 16, st);
 	}
 	qh_launch_sumsq_batched(B, (long)npad, r, sV, part, outs + B, st);
 	qh_launch_sumsq_batched(B, (long)npad, x, sV, part, outs + 2 * B, st);
@@ -626,7 +679,7 @@ static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	SRV_HIPCHK(hipMemcpyAsync(dpos, h + align256(pt.nR * 24) + align256(pt.nK * 16), np * 24, hipMemcpyHostToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(dprm, hp, 11 * 8, hipMemcpyHostToDevice, st));
 	free(pt.R); free(pt.K);
-	SRV_HIPCHK(hipMemsetAsync(tt, 0, np * 4, st));
+	qh_zero(tt, np * 4, st);
 	qh_launch_mie(lMax, 1, dprm, mie, st);
 	qh_launch_expand_tdiag(lMax, 1, mie, td, st);
 	SRV_HIPCHK(hipMemcpyAsync(dk, dprm + 9, 16, hipMemcpyDeviceToDevice, st));
@@ -638,7 +691,7 @@ static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	qh_launch_pair_sigma_total((int)np, nsc, ss, sl, s0, tot, st);
 	qh_launch_e32_assemble_M(qh_tables_get(c), (int)np, nsc, tot, qh_e32_bq_mode(), tt, td, M, (long)L, st);
 	int *info = tt;   /* reuse: np >= 1 ints */
-	SRV_HIPCHK(hipMemsetAsync(info, 0, 4, st));
+	qh_zero(info, 4, st);
 	qh_getrf_batched_ws_f64((int)L, 1, M, 0, ipiv, perm, info, ws, st);
 	qh_launch_logdet((int)L, M, ipiv, outs, st);
 	SRV_HIPCHK(hipMemcpyAsync(outs + 2, info, 4, hipMemcpyDeviceToDevice, st));
@@ -692,11 +745,11 @@ static void launch_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int ch
 	for (int n = 0; n < N; ++n) mie_params(r, u->bcont->z_dz[n][0], hp + 9 * n);
 	SRV_HIPCHK(hipMemcpyAsync(u->bpos, u->pin, np * 24, hipMemcpyHostToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(dprm, hp, N * 9 * 8, hipMemcpyHostToDevice, st));
-	SRV_HIPCHK(hipMemsetAsync(u->btt, 0, np * 4, st));
-	SRV_HIPCHK(hipMemsetAsync(u->dA0, 0, L * l * 16, st));
-	SRV_HIPCHK(hipMemsetAsync(u->dA1, 0, L * l * 16, st));
-	SRV_HIPCHK(hipMemsetAsync(u->dA0c, 0, L * l * 16, st));
-	SRV_HIPCHK(hipMemsetAsync(u->dA1c, 0, L * l * 16, st));
+	qh_zero(u->btt, np * 4, st);
+	qh_zero(u->dA0, L * l * 16, st);
+	qh_zero(u->dA1, L * l * 16, st);
+	qh_zero(u->dA0c, L * l * 16, st);
+	qh_zero(u->dA1c, L * l * 16, st);
 	qh_launch_probe((long)(L * l), r->beyn_seed, u->bV, st);
 	qh_launch_mie(lMax, N, dprm, mie, st);
 	qh_launch_expand_tdiag(lMax, N, mie, u->btd, st);
@@ -720,8 +773,8 @@ static void beyn_enqueue_chunk(qpms_hip_server_t *s, qh_unit *u, hipStream_t st)
 		qh_launch_modeproblem(tb, np, u->bpos, u->btt, u->btd + (n0 + i) * nb * 2, creal(k), cimag(k), M + 2 * i * sM,
 			L, st);
 	}
-	SRV_HIPCHK(hipMemsetAsync(u->binfo, 0, C * 4, st));
-	SRV_HIPCHK(hipMemsetAsync(X, 0, C * L * l * 16, st));
+	qh_zero(u->binfo, C * 4, st);
+	qh_zero(X, C * L * l * 16, st);
 	for (int i = 0; i < C; ++i) qh_copy_c64((long)(L * l), u->bV, R + i * sV * 2, st);
 	if (s->o.mixed) {
 		qh_d2f((long)(C * L * L), M, u->bM32, st);

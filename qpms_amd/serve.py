@@ -60,6 +60,8 @@ def _bind():
     L.qpms_hip_server_create.restype = vp
     L.qpms_hip_server_create.argtypes = [ctypes.POINTER(Opts)]
     L.qpms_hip_server_destroy.argtypes = [vp]
+    L.qpms_hip_server_arena_bytes.restype = ctypes.c_size_t
+    L.qpms_hip_server_arena_bytes.argtypes = [vp]
     L.qpms_hip_server_clock.restype = ctypes.c_double
     L.qpms_hip_server_clock.argtypes = [vp]
     L.qpms_hip_server_submit.restype = ctypes.c_ulonglong
@@ -79,11 +81,19 @@ def _bind():
 
 
 class Server:
-    def __init__(self, arena_gib=64, nstreams=8, mixed=True, batch_max_dim=8192, batch_max=32, refine_steps=3):
+    """arena_gib=None sizes the arena from the device's free memory."""
+
+    def __init__(self, arena_gib=None, nstreams=8, mixed=True, batch_max_dim=8192, batch_max=32, refine_steps=3):
         self.L = _bind()
-        o = Opts(int(arena_gib * (1 << 30)), nstreams, int(mixed), batch_max_dim, batch_max, refine_steps)
+        nbytes = 0 if arena_gib is None else int(arena_gib * (1 << 30))
+        o = Opts(nbytes, nstreams, int(mixed), batch_max_dim, batch_max, refine_steps)
         self.h = self.L.qpms_hip_server_create(ctypes.byref(o))
+        if not self.h:
+            raise MemoryError("qpms_hip_server_create failed (see stderr)")
         self._keep = {}
+
+    def arena_bytes(self):
+        return self.L.qpms_hip_server_arena_bytes(self.h)
 
     def submit(self, req):
         rid = self.L.qpms_hip_server_submit(self.h, ctypes.byref(req.c))

@@ -224,3 +224,91 @@ def scattered_E_alt(lMax, pos, k, cvf, where, J=3):
     o.or_scattered_E_alt(c.c, len(pos), pos.ctypes.data, c_cplx.of(k), J, cvf.ctypes.data, len(where),
                          where.ctypes.data, out.ctypes.data)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Host LAPACK (the reference's own dependency for the LU: LAPACKE_zgetrf /
+# zgetrs at scatsystem.c:2214, 2270). Intel MKL's single dynamic library when
+# present (this image ships it), else scipy's bundled LAPACK.
+# ---------------------------------------------------------------------------
+MKL_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so.1", "libmkl_rt.so"]
+_lap = None
+
+
+class HostLapack:
+    """zgetrf / zgetrs on the host, row-major, with a settable thread count."""
+
+    def __init__(self):
+        self.name, self.mkl = None, None
+        for p in MKL_CANDIDATES:
+            try:
+                L = ctypes.CDLL(p)
+            except OSError:
+                continue
+            if hasattr(L, "LAPACKE_zgetrf") and hasattr(L, "LAPACKE_zgetrs"):
+                L.LAPACKE_zgetrf.restype = ctypes.c_int
+                L.LAPACKE_zgetrf.argtypes = [i32, i32, i32, vp, i32, vp]
+                L.LAPACKE_zgetrs.restype = ctypes.c_int
+                L.LAPACKE_zgetrs.argtypes = [i32, ctypes.c_char, i32, i32, vp, i32, vp, vp, i32]
+                L.MKL_Set_Num_Threads.argtypes = [i32]
+                L.MKL_Get_Max_Threads.restype = i32
+                self.mkl, self.name = L, f"Intel MKL ({p}), LAPACKE_zgetrf/zgetrs"
+                break
+        if self.mkl is None:
+            import scipy.linalg.lapack as sl
+            self.sl = sl
+            self.name = "scipy.linalg.lapack zgetrf/zgetrs (bundled OpenBLAS)"
+
+    def set_threads(self, n):
+        if self.mkl is not None:
+            self.mkl.MKL_Set_Num_Threads(int(n))
+        else:   # OpenBLAS reads its thread count at load; nothing to do per call
+            pass
+
+    def zgetrf(self, A):
+        """LU of A (complex128, C-contiguous), in place with MKL; returns
+        (LU, ipiv 1-based, info)."""
+        n = A.shape[0]
+        if self.mkl is not None:
+            ipiv = np.zeros(n, np.int32)
+            info = self.mkl.LAPACKE_zgetrf(101, n, n, A.ctypes.data, n, ipiv.ctypes.data)   # LAPACK_ROW_MAJOR
+            return A, ipiv, info
+        lu, piv, info = self.sl.zgetrf(A)
+        return lu, piv.astype(np.int32) + 1, info
+
+    def zgetrs(self, LU, ipiv, b):
+        """Solve with one right-hand side (row-major, ldb = 1 as the reference passes it)."""
+        n = LU.shape[0]
+        x = np.ascontiguousarray(b, np.complex128).copy()
+        if self.mkl is not None:
+            info = self.mkl.LAPACKE_zgetrs(101, b"N", n, 1, LU.ctypes.data, n, ipiv.ctypes.data, x.ctypes.data, 1)
+        else:
+            x, info = self.sl.zgetrs(LU, ipiv - 1, x)
+        assert info == 0
+        return x
+
+
+def host_lapack():
+    global _lap
+    if _lap is None:
+        _lap = HostLapack()
+    return _lap
+
+
+def reference_cpu_solve(pos, lMax, radius, eps_in, eps_bg, omega, kdir, E, lap):
+    """The reference's CPU path of one plane-wave solve, as a caller of the
+    reference library runs it (misc/finiterectlat-scatter.py:80-111 through
+    scatsystem.c:1283-1343, 2202-2273): T from Mie, M = I - T W built by the
+    oracle's pair loop (threads as set by or_set_threads), LU and solve by host
+    LAPACK, extinction -Re(a^H f)/k^2. Returns sigma_ext."""
+    tdiag, k = mie_tdiag(lMax, radius, eps_in, eps_bg, omega)
+    oc = OracleCalc(lMax)
+    pos = np.ascontiguousarray(pos, np.float64)
+    n = len(pos)
+    M = oc.build_M(pos, np.zeros(n, np.int32), tdiag, k)
+    a = planewave(lMax, pos, k.real * np.asarray(kdir, float), E).ravel()
+    Ta = np.tile(tdiag, n) * a
+    LU, ipiv, info = lap.zgetrf(M)
+    assert info == 0
+    f = lap.zgetrs(LU, ipiv, Ta)
+    return -np.vdot(a, f).real / (k * k.conjugate()).real
