@@ -205,8 +205,9 @@ def measure_peaks(L, ev, stream):
 
 
 def measure_lu(L, ev, stream, n_part=546, reps=3):
-    """FP32 LU of a mode-problem matrix of order 30 * n_part = 16380 (the
-    serving loop's dominant work), HIP events on the library stream."""
+    """FP32 LU of a mode-problem matrix of order 30 * n_part (16380 by
+    default; the serving loop's dominant work), HIP events on the library
+    stream."""
     import qpms_amd
     rng = np.random.default_rng(2)
     n = 30 * n_part
@@ -245,8 +246,8 @@ def measure_lu(L, ev, stream, n_part=546, reps=3):
             "frac": ach / FP32_MFMA_PEAK,
             # HBM bytes per factorisation from the two rocprofv3 --pmc passes of
             # this probe (FETCH_SIZE, WRITE_SIZE, copy-kernel calibration)
-            "traffic": profile_traffic("lu_fp32_order16380"),
-            "kernel": "FP32 LU, order 16380 (tournament panels + trailing cgemm)",
+            "traffic": profile_traffic("lu_fp32_order16380") if n == 16380 else None,
+            "kernel": f"FP32 LU, order {n} (tournament panels + trailing GEMM)",
             "avg_launch_ms": avg, "algorithmic_flops_per_launch": flops}
 
 
@@ -258,10 +259,11 @@ def meminfo_gib():
     return f.value / 2 ** 30, t.value / 2 ** 30
 
 
-def replay(srv, reqs, label):
-    """Submit reqs at their arrival times (relative to the first one) and run
-    the loop until every request is answered. Returns ({index: (response,
-    seconds from the first submission)}, wall seconds)."""
+def replay(srv, reqs, label, burst=False):
+    """Submit reqs at their arrival times (relative to the first one; all at
+    once with burst) and run the loop until every request is answered.
+    Returns ({index: (response, seconds from the first submission)}, wall
+    seconds)."""
     t_first = reqs[0].arrival
     id2i, got = {}, {}
     nxt = 0
@@ -269,7 +271,7 @@ def replay(srv, reqs, label):
     last_log = t0
     while len(got) < len(reqs):
         now = time.perf_counter() - t0
-        while nxt < len(reqs) and reqs[nxt].arrival - t_first <= now:
+        while nxt < len(reqs) and (burst or reqs[nxt].arrival - t_first <= now):
             id2i[srv.submit(reqs[nxt])] = nxt
             nxt += 1
         srv.progress(wait=False)
@@ -292,18 +294,21 @@ def rel_diff(a, b, req):
 
 
 def plain_estimate_s(req):
-    """Conservative seconds of one plain-path request, from its order: FP64
-    LUs at 30 TFLOP/s, the host round trips of the mode-problem matrix at
-    8 GB/s each way, and a fixed cost (used only to decide whether a request
-    can still finish inside the plain path's budget)."""
+    """Seconds of one plain-path request from its order (used only to decide
+    whether it can still finish inside the plain path's budget): FP64 LUs at
+    30 TFLOP/s and the host round trips of the matrix through pageable
+    buffers at 5.5 GB/s (a plane-wave solve moves it four times: build,
+    factorise in and out, solve; a contour node twice), plus fixed costs.
+    Calibrated on the MI355X box: a plane-wave solve of order 14,790 takes
+    2.9 s."""
     from qpms_amd import serve as sv
     L = float(req.dim)
-    one = 8.0 / 3.0 * L ** 3 / 30e12 + 3 * 16.0 * L * L / 8e9 + 0.05
+    lu = 8.0 / 3.0 * L ** 3 / 30e12
     if req.kind == sv.KIND_BEYN:
-        return req.c.beyn_nodes * one
+        return req.c.beyn_nodes * (lu + 32.0 * L * L / 5.5e9 + 0.02)
     if req.kind == sv.KIND_PERIODIC:
-        return 2 * one + 0.5
-    return one
+        return lu + 32.0 * L * L / 5.5e9 + 0.5
+    return lu + 64.0 * L * L / 5.5e9 + 0.05
 
 
 def cpu_model():
@@ -337,8 +342,10 @@ def cpu_baseline_scatter(reqs, served, budget_s):
         done, t_used, dims = 0, 0.0, []
         for i in pw:
             r = reqs[i]
-            if t_used > 0 and t_used + 8.0 / 3.0 * float(r.dim) ** 3 / (5e9 * threads) > budget_s:
-                break   # this order alone would overrun the budget
+            # host seconds: the oracle's pair loop (~25 us per particle pair) and the LU (~12 GFLOP/s a thread)
+            est = (float(r.c.npart) ** 2 * 25e-6 + 8.0 / 3.0 * float(r.dim) ** 3 / 12e9) / threads
+            if t_used + est > budget_s:
+                continue   # this order alone would overrun what is left of the budget
             t0 = time.perf_counter()
             se = oracle_ffi.reference_cpu_solve(r.pos, r.c.lMax, r.c.radius, complex(r.c.eps_in.re, r.c.eps_in.im),
                                                 r.c.eps_bg, r.c.omega, (0, 0, 1), np.array([1, 0, 0], complex), lap)
@@ -355,9 +362,10 @@ def cpu_baseline_scatter(reqs, served, budget_s):
     lap.set_threads(cores)
     out["value"] = out["all"]["value"]
     out["cores"] = cores
-    out["sample"] = (f"the reference's CPU path on the first plane-wave requests of the timed segment in stream order "
-                     f"({out['all']['requests']} on {cores} threads, {out['one']['requests']} on 1 thread, "
-                     f"{budget_s:.0f} s budget each); the GPU value covers every request kind and order")
+    out["sample"] = (f"the reference's CPU path on the plane-wave requests of the timed segment in stream order, "
+                     f"skipping those too large for what is left of the {budget_s:.0f} s budget "
+                     f"({out['all']['requests']} on {cores} threads, {out['one']['requests']} on 1 thread); the GPU "
+                     f"value covers every request kind and order")
     out["served_vs_cpu_max_rel_diff"] = float(max(diffs)) if diffs else None
     return out
 
@@ -644,6 +652,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-wbuild", action="store_true")
     ap.add_argument("--lu-probe", action="store_true", help="only the FP32 LU measurement (for rocprofv3)")
+    ap.add_argument("--lu-particles", type=int, default=546,
+                    help="particles (lMax 3) of the LU probe's matrix: order 30 x this (546: 16380)")
     ap.add_argument("--w-probe", action="store_true",
                     help="one 2 GiB copy kernel (counter calibration) and one N = 2048 W build (for rocprofv3 --pmc)")
     ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
@@ -666,7 +676,7 @@ def main():
         w_probe(L)
         return
     if args.lu_probe:
-        print(json.dumps(measure_lu(L, ev, stream, reps=2)), flush=True)
+        print(json.dumps(measure_lu(L, ev, stream, n_part=args.lu_particles, reps=2)), flush=True)
         return
     if args.burst:
         print(json.dumps(burst(args.burst)), flush=True)
@@ -722,35 +732,38 @@ def main():
 
     plain = None
     if not args.no_plain:
-        times, errs, kinds_p = [], [], []
+        times, errs, kinds_p, covered, excluded = [], [], [], [], []
         t_start = time.perf_counter()
-        stopped_at = None
         for i, r in enumerate(timed):
             left = args.plain_budget - (time.perf_counter() - t_start)
-            if plain_estimate_s(r) > left:
-                stopped_at = i
+            if left <= 0:
+                excluded += list(range(i, len(timed)))
                 break
+            if plain_estimate_s(r) > left:   # cannot finish inside what is left of the budget
+                excluded.append(i)
+                continue
             log(f"plain FP64: request {i} kind {r.kind} order {r.dim}")
             t0 = time.perf_counter()
             pr = sv.request_plain(r)
             times.append(time.perf_counter() - t0)
             errs.append(rel_diff(got[i][0], pr, r))
             kinds_p.append(r.kind)
+            covered.append(i)
         P = len(times)
         if P:
             if P == len(timed):
                 served_wall, served_src = wall, "the timed replay"
-            else:   # the serving loop on exactly the plain path's requests
+            else:   # the serving loop on exactly the plain path's requests, submitted at once as the
+                # plain path takes them back to back
                 L.qpms_hip_sync()
-                got_p, served_wall = replay(srv, timed[:P], "served prefix")
+                got_p, served_wall = replay(srv, [timed[i] for i in covered], "served subset", burst=True)
                 L.qpms_hip_sync()
-                served_src = f"a replay of requests 0..{P - 1}"
+                served_src = f"the {P} compared requests submitted at once"
             plain = {"requests": P, "of_timed": len(timed), "plain_busy_s": float(sum(times)),
                      "solves_per_s": P / sum(times), "served_wall_s": served_wall, "served_solves_per_s": P / served_wall,
                      "served_measured_by": served_src,
-                     "stopped_before": None if stopped_at is None else
-                     {"request": stopped_at, "kind": timed[stopped_at].kind, "order": timed[stopped_at].dim,
-                      "estimated_plain_s": plain_estimate_s(timed[stopped_at])},
+                     "excluded": [{"request": i, "kind": timed[i].kind, "order": timed[i].dim,
+                                   "estimated_plain_s": round(plain_estimate_s(timed[i]), 1)} for i in excluded],
                      "busy_s_by_kind": {str(k): float(sum(t for t, kk in zip(times, kinds_p) if kk == k))
                                         for k in (0, 1, 2)},
                      "p50_s": float(np.percentile(times, 50)), "p99_s": float(np.percentile(times, 99)),
@@ -758,7 +771,7 @@ def main():
                      "max_rel_diff_by_kind": {str(k): float(max([e for e, kk in zip(errs, kinds_p) if kk == k] or [0]))
                                               for k in (0, 1, 2)}}
             log(f"plain FP64: {P} requests in {sum(times):.1f} s busy; served {served_wall:.1f} s on the same; "
-                f"max rel diff {plain['max_rel_diff_served_vs_fp64']:.2e}")
+                f"excluded {excluded}; max rel diff {plain['max_rel_diff_served_vs_fp64']:.2e}")
     srv.close()
     log(f"device memory after the server: {meminfo_gib()[0]:.1f} GiB free")
 

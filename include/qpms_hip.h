@@ -680,6 +680,17 @@ int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_dev, void *s
 int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, const int *ipiv_dev,
 		_Complex double *B_dev, void *stream);
 
+/* Autotuner of the FP32 LU's trailing update (no reference counterpart):
+ * times the library GEMM and the hand-written kernels (3M MFMA kernel in
+ * three tile shapes, four-product kernel) per size class on scratch operands
+ * (allocated and freed here) and records the fastest for every later LU in
+ * the process. Returns the number of classes tuned; `report` (optional,
+ * 2 x 6 x 5 floats) receives the milliseconds [batched][class][variant].
+ * The serving loop runs it at creation when QPMS_HIP_AUTOTUNE is set. */
+int qpms_hip_autotune(float *report);
+/* The tuned table as text ("batched class-dim variant" per line). */
+int qpms_hip_autotune_table(char *buf, size_t len);
+
 /* Batched Ewald sums: ntrip (beta, k, s) triples on one 2D lattice in the xy
  * plane. triples_dev holds 8 doubles per triple {beta_x, beta_y, beta_z,
  * Re k, Im k, s_x, s_y, s_z}; R_dev nR direct-lattice points (x, y, z), the

@@ -45,6 +45,31 @@ __global__ void k_beyn_accum(long n, const v2d *__restrict__ X, double dzr, doub
   }
 }
 
+// The same accumulation for node *cursor + slot of a node table (8 doubles
+// per node: k re, k im, dz re, dz im, (z - z0) re, (z - z0) im, coarse flag,
+// unused), so that a chunk's launches do not depend on its first node.
+__global__ void k_beyn_accum_node(long n, const v2d *__restrict__ X, const double *__restrict__ ntab,
+                                  const int *__restrict__ cursor, int slot, v2d *__restrict__ A0,
+                                  v2d *__restrict__ A1, v2d *__restrict__ A0c, v2d *__restrict__ A1c) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *nd = ntab + 8 * (size_t)(cursor[0] + slot);
+  const cplx x = ld2(X[i]);
+  const cplx a = mk(nd[2], nd[3]) * x;
+  const cplx b = mk(nd[4], nd[5]) * a;
+  A0[i] = st2(ld2(A0[i]) + a);
+  A1[i] = st2(ld2(A1[i]) + b);
+  if (nd[6] != 0.0) {
+    A0c[i] = st2(ld2(A0c[i]) + 2.0 * a);
+    A1c[i] = st2(ld2(A1c[i]) + 2.0 * b);
+  }
+}
+
+// *cursor += by (the next chunk's first node), one thread.
+__global__ void k_cursor_advance(int *cursor, int by) {
+  if (threadIdx.x == 0) cursor[0] += by;
+}
+
 // out (l x m) = transpose of in (m x l)
 __global__ void k_transpose(int m, int l, const v2d *__restrict__ in, v2d *__restrict__ out) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -346,6 +371,20 @@ extern "C" int qh_beyn_accum(long n, const double *X, double dzr, double dzi, do
                              double *A0, double *A1, double *A0c, double *A1c, void *stream) {
   QH_LAUNCH(k_beyn_accum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
                      (const v2d *)X, dzr, dzi, wr, wi, coarse, (v2d *)A0, (v2d *)A1, (v2d *)A0c, (v2d *)A1c);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_beyn_accum_node(long n, const double *X, const double *ntab, const int *cursor, int slot,
+                                  double *A0, double *A1, double *A0c, double *A1c, void *stream) {
+  QH_LAUNCH(k_beyn_accum_node, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)qh_stream(stream), n,
+            (const v2d *)X, ntab, cursor, slot, (v2d *)A0, (v2d *)A1, (v2d *)A0c, (v2d *)A1c);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qh_cursor_advance(int *cursor, int by, void *stream) {
+  QH_LAUNCH(k_cursor_advance, dim3(1), dim3(64), 0, (hipStream_t)qh_stream(stream), cursor, by);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -875,11 +875,27 @@ extern "C" int qh_stream_state_count(void) {
   return g_ntab;
 }
 
+static bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 template <class T>
 static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::t *A, long lda, long sA,
                           const typename C2<T>::t *B, long ldb, long sB, typename C2<T>::t *C, long ldc, long sC,
                           hipStream_t s, bool own_only = false) {
   if (M <= 0 || N <= 0 || K <= 0) return;
+  // inside a captured graph only hand-written kernels (a library call may
+  // record allocations or memsets into the graph)
+  const bool capturing = stream_capturing(s);
+  if (capturing) own_only = true;
+  if (sizeof(T) == 4 && (!own_only || capturing)) {   // FP32: the variant the autotuner picked for this size class
+    int v = qh_lu_gemm_variant(M, N, batch > 1);
+    if (v == 0 && capturing) v = qh_lu_gemm_best_own(M, N, batch > 1);
+    qh_lu_gemm_run(v, M, N, K, batch, (const float *)A, lda, sA, (const float *)B, ldb, sB, (float *)C, ldc, sC,
+                   v == 0 ? (void *)blas_for(s) : nullptr, s);
+    return;
+  }
   if (own_only || !use_rocblas() || (long)M * N < 512L * 512) {
     QH_LAUNCH(k_gemm_sub<T>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0, s, M, N, K, A,
                        lda, sA, B, ldb, sB, C, ldc, sC);
@@ -914,6 +930,26 @@ static bool use_lookahead() {
     v = !(e && !strcmp(e, "0"));
   }
   return v == 1;
+}
+
+// The tournament kernels stage a chunk of more than 64 KB of LDS: raise
+// their dynamic-LDS limit once (before any launch sequence is captured).
+template <class T>
+static void ts_prepare() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    HIPCHK(hipFuncSetAttribute((const void *)k_ts_local<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
+    HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
+  });
+}
+
+extern "C" void qh_cgemm3m_prepare(void);
+// Every kernel attribute the LU paths set, set now (the serving loop calls
+// this at creation, before it captures any launch sequence into a graph).
+extern "C" void qh_lu_prepare(void) {
+  ts_prepare<float>();
+  ts_prepare<double>();
+  qh_cgemm3m_prepare();
 }
 
 // Workspace bytes for getrf_batched<T>(n, batch).
@@ -954,11 +990,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     ws = own;
   }
   const dim3 bts(TS_THREADS);
-  static std::once_flag lds_once;   // the tournament kernels stage a chunk of more than 64 KB
-  std::call_once(lds_once, [] {
-    HIPCHK(hipFuncSetAttribute((const void *)k_ts_local<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
-    HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
-  });
+  ts_prepare<T>();
   char *bufA = ws, *bufB = ws + (long)Gmax * setb;   // two candidate buffers per matrix
   char *fin0 = ws + 2 * (long)Gmax * setb;           // per-strip slots of the last level's output
   const long finb = (long)PanelWS<T>::fin_bytes();
@@ -1000,7 +1032,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // update has reached block k+1's columns, while the rest of that update
   // runs on `s`. The interchanges of a block are applied to the columns
   // outside it afterwards, on `s` (the strips touch only their own block).
-  const bool la = lookahead && use_lookahead() && n > 2 * LU_NB;
+  const bool la = lookahead && use_lookahead() && n > 2 * LU_NB && !stream_capturing(s);
   LaCtx *cx = la ? la_for(s) : nullptr;
   hipStream_t sp = la ? cx->side : s;
   if (perm) QH_LAUNCH(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
@@ -1272,6 +1304,21 @@ extern "C" int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int 
                                        void *stream) {
   getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
                        false);
+  return 0;
+}
+// The same with the outer blocks' strips on a lookahead side stream of
+// `stream` when `lookahead` (large orders: the strip chain then overlaps the
+// trailing GEMM; the side stream belongs to `stream` until released).
+extern "C" int qh_getrf_batched_wsla_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info,
+                                         void *ws, int lookahead, void *stream) {
+  getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
+                       lookahead != 0);
+  return 0;
+}
+extern "C" int qh_getrf_batched_wsla_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info,
+                                         void *ws, int lookahead, void *stream) {
+  getrf_batched<double>(n, batch, (v2d *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
+                        lookahead != 0);
   return 0;
 }
 extern "C" int qh_getrs_batched_f64(int n, int nrhs, int batch, const double *LU, long sA, const int *perm,

@@ -208,13 +208,18 @@ __device__ __forceinline__ void w_rows(const PairData<LMAX> &pd, const v2d *__re
   (w_row<LMAX, MODE_M, Ys>(pd, mA, mB, lds, lane, R, Rloc, C0, nvalid, tdiagR, W, ld), ...);
 }
 
+// With `node` set (Beyn quadrature chunks replayed as one graph), the
+// wavenumber is node[0..1] and the T diagonals are tdiag + node_td * nb for
+// the node entry node = ntab + 8 (*cursor + slot): the launch itself does not
+// depend on which node it builds.
 template <int LMAX, bool MODE_M>
 __global__ __launch_bounds__(64) void k_w_build(const double *__restrict__ pos, int npart, int row_p0,
                                                 int nchunks, double k_re, double k_im, int kind,
                                                 const v2d *__restrict__ mA, const v2d *__restrict__ mB,
                                                 const int *__restrict__ ttype,
                                                 const v2d *__restrict__ tdiag, v2d *__restrict__ W,
-                                                size_t ld) {
+                                                size_t ld, const double *__restrict__ ntab,
+                                                const int *__restrict__ cursor, int slot) {
   constexpr int NE = Dims<LMAX>::NE, NB = Dims<LMAX>::NB;
   __shared__ v2d lds[64 * (NB + 1)];
   const int lane = threadIdx.x;
@@ -229,6 +234,12 @@ __global__ __launch_bounds__(64) void k_w_build(const double *__restrict__ pos, 
     dx = rx - pos[3 * C];
     dy = ry - pos[3 * C + 1];
     dz = rz - pos[3 * C + 2];
+  }
+  if (ntab) {
+    const double *nd = ntab + 8 * (size_t)(cursor[0] + slot);
+    k_re = nd[0];
+    k_im = nd[1];
+    tdiag += (size_t)(cursor[0] + slot) * NB;
   }
   PairData<LMAX> pd;
   pair_setup_cart<LMAX>(dx, dy, dz, mk(k_re, k_im), kind, pd);
@@ -295,7 +306,8 @@ extern "C" int qh_launch_ab_batch(const qh_trans_tables *t, size_t npts, const d
 template <bool MODE_M>
 static void launch_w(const qh_trans_tables *t, size_t npart, const double *pos_dev, double k_re,
                      double k_im, int kind, size_t row_p0, size_t row_np, const int *ttype,
-                     const double *tdiag, double *W_dev, size_t ld, hipStream_t s) {
+                     const double *tdiag, double *W_dev, size_t ld, hipStream_t s,
+                     const double *ntab = nullptr, const int *cursor = nullptr, int slot = 0) {
   const int nchunks = (int)((npart + 63) / 64);
   const size_t nblk = row_np * (size_t)nchunks;
   if (nblk == 0) return;
@@ -306,7 +318,7 @@ static void launch_w(const qh_trans_tables *t, size_t npart, const double *pos_d
   v2d *W = (v2d *)W_dev;
   switch (t->lMax) {
 #define QH_W_CASE(LM) case LM: QH_LAUNCH((k_w_build<LM, MODE_M>), grid, block, 0, s, pos_dev, (int)npart, \
-      (int)row_p0, nchunks, k_re, k_im, kind, mA, mB, ttype, td, W, ld); break;
+      (int)row_p0, nchunks, k_re, k_im, kind, mA, mB, ttype, td, W, ld, ntab, cursor, slot); break;
     QH_W_CASE(1) QH_W_CASE(2) QH_W_CASE(3) QH_W_CASE(4)
 #undef QH_W_CASE
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "lMax %d not instantiated (1..4)", t->lMax);
@@ -330,6 +342,17 @@ extern "C" int qh_launch_modeproblem(const qh_trans_tables *t, size_t npart, con
   if (ld < npart * (size_t)(2 * t->nelem)) qpms_hip_die(__FILE__, __LINE__, __func__, "leading dimension too small");
   launch_w<true>(t, npart, pos_dev, k_re, k_im, QPMS_HANKEL_PLUS, 0, npart, ttype_dev, tdiag_dev, M_dev, ld,
                  (hipStream_t)qh_stream(stream));
+  return 0;
+}
+
+// The mode problem of Beyn node *cursor + slot (wavenumber from the node
+// table ntab, 8 doubles per node; T diagonals at tdiag + node * nb).
+extern "C" int qh_launch_modeproblem_node(const qh_trans_tables *t, size_t npart, const double *pos_dev,
+                                          const int *ttype_dev, const double *tdiag_dev, const double *ntab,
+                                          const int *cursor, int slot, double *M_dev, size_t ld, void *stream) {
+  if (ld < npart * (size_t)(2 * t->nelem)) qpms_hip_die(__FILE__, __LINE__, __func__, "leading dimension too small");
+  launch_w<true>(t, npart, pos_dev, 0.0, 0.0, QPMS_HANKEL_PLUS, 0, npart, ttype_dev, tdiag_dev, M_dev, ld,
+                 (hipStream_t)qh_stream(stream), ntab, cursor, slot);
   return 0;
 }
 

@@ -78,6 +78,15 @@ int qh_getrs_batched_f32(int n, int nrhs, int batch, const float *LU, long sA, c
 		float *B, long sB, float *work, void *stream);
 int qh_gemm_sub_f64(int M, int N, int K, int batch, const double *A, long lda, long sA, const double *B,
 		long ldb, long sB, double *C, long ldc, long sC, void *stream);
+int qh_gemm_sub_f32(int M, int N, int K, int batch, const float *A, long lda, long sA, const float *B,
+		long ldb, long sB, float *C, long ldc, long sC, void *stream);
+/* k_gemm.hip: the 3M FP32 complex GEMM and the tuned LU trailing update */
+int qh_cgemm3m_sub(int M, int N, int K, int batch, const float *A, long lda, long sA, const float *B,
+		long ldb, long sB, float *C, long ldc, long sC, int tile, void *stream);
+int qh_lu_gemm_variant(int M, int N, int batched);
+int qh_lu_gemm_best_own(int M, int N, int batched);
+int qh_lu_gemm_run(int v, int M, int N, int K, int batch, const float *A, long lda, long sA,
+		const float *B, long ldb, long sB, float *C, long ldc, long sC, void *blas, void *stream);
 int qh_d2f(long n, const double *x, float *y, void *stream);
 int qh_f2d_add(long n, const float *d, double *x, void *stream);
 int qh_copy_c64(long n, const double *x, double *y, void *stream);

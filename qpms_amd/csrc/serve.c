@@ -59,6 +59,16 @@ int qh_getrf_batched_ws_f64(int n, int batch, double *A, long sA, int *ipiv, int
 		void *stream);
 int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info, void *ws,
 		void *stream);
+int qh_getrf_batched_wsla_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info, void *ws,
+		int lookahead, void *stream);
+int qh_getrf_batched_wsla_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info, void *ws,
+		int lookahead, void *stream);
+/* Orders from which a served LU runs its strips on a lookahead side stream
+ * (the strip chain then overlaps the trailing GEMM), and up to which Beyn
+ * chunks are replayed from a captured graph (launch-bound orders; larger
+ * ones are compute-bound and keep the lookahead, which a capture excludes). */
+#define SRV_LOOKAHEAD_MIN 4096
+#define SRV_GRAPH_MAX 12288
 int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
 		double *A0, double *A1, double *A0c, double *A1c, void *stream);
 int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
@@ -184,6 +194,9 @@ typedef struct {
 	int *bipiv, *bperm, *binfo, *btt;
 	void *bws, *btail;
 	beyn_contour_t *bcont;
+	double *btab;          /* node table: k, dz, z - z0, coarse flag (8 doubles per node) */
+	int *bcur;             /* device cursor: first node of the chunk being run */
+	hipGraphExec_t gexec;  /* one chunk's launches, captured on the first chunk */
 	int active;
 } qh_unit;
 
@@ -202,6 +215,8 @@ struct qpms_hip_server_t {
 	int nready, capready;
 	qh_job **order;        /* launch-order scratch */
 	double deadline;       /* 0, or: start only what is estimated to finish by then */
+	int graphs;            /* Beyn chunks replayed from a captured hipGraph (QPMS_HIP_GRAPHS != 0) */
+	int beyn_chunk_max;    /* most nodes per chunk (QPMS_HIP_BEYN_CHUNK, 0: no limit) */
 	size_t order_cap;
 	unsigned long long next_id;
 	struct timespec t0;
@@ -262,7 +277,8 @@ static size_t beyn_bytes(const qpms_hip_server_t *s, size_t L, size_t l, int chu
 	const size_t nb = 2 * (size_t)lMax * (lMax + 2);
 	return chunk * beyn_node_bytes(s, L, l) + align256(qh_getrf_ws_bytes(s->o.mixed ? 32 : 64, (int)L, chunk))
 		+ 5 * align256(L * l * 16) + align256(L * 24) + align256(qh_beyn_ws_bytes(L, l))
-		+ align256(N * 9 * 8) + align256(N * 2 * (lMax + 1) * 16) + align256(N * nb * 16) + 65536 + chunk * 4096;
+		+ align256(N * 9 * 8) + align256(N * 2 * (lMax + 1) * 16) + align256(N * nb * 16) + align256(N * 64) + 256
+		+ 65536 + chunk * 4096;
 }
 
 static const qpms_trans_calculator *srv_calc(qpms_hip_server_t *s, int lMax) {
@@ -298,6 +314,12 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 	if (qpms_hip_init() != 0) {
 		fprintf(stderr, "qpms_hip_server_create: no HIP device\n");
 		return NULL;
+	}
+	static int tuned = 0;   /* the GEMM autotuner runs once per process, before the arena takes the memory */
+	const char *et = getenv("QPMS_HIP_AUTOTUNE");
+	if (!tuned && et && strcmp(et, "0")) {
+		qpms_hip_autotune(NULL);
+		tuned = 1;
 	}
 	qpms_hip_server_t *s;
 	QH_CALLOC(s, 1, sizeof(*s));
@@ -339,6 +361,10 @@ qpms_hip_server_t *qpms_hip_server_create(const qpms_hip_server_opts_t *opts) {
 		s->calc[l] = qpms_trans_calculator_init(l, QPMS_NORMALISATION_DEFAULT);
 		qh_tables_get(s->calc[l]);   /* device tables now, not inside the loop */
 	}
+	const char *eg = getenv("QPMS_HIP_GRAPHS"), *ec = getenv("QPMS_HIP_BEYN_CHUNK");
+	s->graphs = !(eg && !strcmp(eg, "0"));
+	s->beyn_chunk_max = ec ? atoi(ec) : 0;
+	qh_lu_prepare();   /* kernel attributes set before any launch sequence is captured */
 	s->next_id = 1;
 	clock_gettime(CLOCK_MONOTONIC, &s->t0);
 	return s;
@@ -352,6 +378,7 @@ void qpms_hip_server_destroy(qpms_hip_server_t *s) {
 	for (int u = 0; u < MAX_UNITS; ++u)
 		if (s->unit[u].active) {
 			SRV_HIPCHK(hipEventDestroy(s->unit[u].done));
+			if (s->unit[u].gexec) SRV_HIPCHK(hipGraphExecDestroy(s->unit[u].gexec));
 			free(s->unit[u].bcont);
 			for (int j = 0; j < s->unit[u].njobs; ++j) { free(s->unit[u].jobs[j]->pos); free(s->unit[u].jobs[j]); }
 		}
@@ -382,7 +409,10 @@ static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_c
 			const size_t l = (size_t)j->req.beyn_l;
 			int c = j->req.beyn_nodes;
 			const int N = j->req.beyn_nodes, lm = j->req.lMax;
-			while (c > 1 && beyn_bytes(s, L, l, c, N, lm) > s->arena.size / 2) --c;
+			/* the largest divisor of N that fits half the arena: every chunk then has
+			 * the same launch sequence (one captured graph serves them all) */
+			while (c > 1 && (N % c || beyn_bytes(s, L, l, c, N, lm) > s->arena.size / 2 ||
+					(s->beyn_chunk_max > 0 && c > s->beyn_chunk_max))) --c;
 			if (beyn_chunk) *beyn_chunk = c;
 			return beyn_bytes(s, L, l, c, N, lm);
 		}
@@ -535,7 +565,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	qh_launch_sumsq_batched(B, (long)nn, M, sM, part, outs + 3 * B, st);
 	if (s->o.mixed) {
 		qh_d2f((long)(B * nn), M, M32, st);
-		qh_getrf_batched_ws_f32((int)npad, B, M32, sM, ipiv, perm, info, ws, st);
+		qh_getrf_batched_wsla_f32((int)npad, B, M32, sM, ipiv, perm, info, ws, npad >= SRV_LOOKAHEAD_MIN, st);
 		qh_copy_c64((long)(B * npad), b, r, st);
 		for (int it = 0; it < s->o.refine_steps; ++it) {
 			qh_d2f((long)(B * npad), r, r32, st);
@@ -547,7 +577,7 @@ static void launch_scatter(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	} else {
 		/* FP64 LU in place on a copy would need another L^2: factor M itself,
 		 * so the residual uses the factors' reconstruction bound instead. */
-		qh_getrf_batched_ws_f64((int)npad, B, M, sM, ipiv, perm, info, ws, st);
+		qh_getrf_batched_wsla_f64((int)npad, B, M, sM, ipiv, perm, info, ws, npad >= SRV_LOOKAHEAD_MIN, st);
 		qh_copy_c64((long)(B * npad), b, x, st);
 		qh_getrs_batched_f64((int)npad, 1, B, M, sM, perm, x, sV, work, st);
 		qh_zero(r, B * npad * 16, st);
@@ -692,7 +722,7 @@ static void launch_periodic(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
 	qh_launch_e32_assemble_M(qh_tables_get(c), (int)np, nsc, tot, qh_e32_bq_mode(), tt, td, M, (long)L, st);
 	int *info = tt;   /* reuse: np >= 1 ints */
 	qh_zero(info, 4, st);
-	qh_getrf_batched_ws_f64((int)L, 1, M, 0, ipiv, perm, info, ws, st);
+	qh_getrf_batched_wsla_f64((int)L, 1, M, 0, ipiv, perm, info, ws, L >= SRV_LOOKAHEAD_MIN, st);
 	qh_launch_logdet((int)L, M, ipiv, outs, st);
 	SRV_HIPCHK(hipMemcpyAsync(outs + 2, info, 4, hipMemcpyDeviceToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(u->pin + in_bytes, outs, 24, hipMemcpyDeviceToHost, st));
@@ -735,16 +765,31 @@ static void launch_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int ch
 	double *dprm = bump(&bp, N * 9 * 8), *mie = bump(&bp, N * 2 * (lMax + 1) * 16);
 	u->btd = bump(&bp, N * nb * 16);
 	u->btail = bump(&bp, qh_beyn_ws_bytes(L, l));
-	/* staged: positions and the Mie parameters of every node */
-	const size_t in_bytes = align256(np * 24) + align256(N * 9 * 8);
+	u->btab = bump(&bp, N * 64);
+	u->bcur = bump(&bp, 256);
+	/* staged: positions, the Mie parameters and the table entry of every node */
+	const size_t in_bytes = align256(np * 24) + align256(N * 9 * 8) + align256(N * 64);
 	u->pin_bytes = in_bytes + 64;
 	pin_take(s, u);
 	memcpy(u->pin, jb->pos, np * 24);
 	u->bcont = beyn_contour_ellipse(CMPLX(r->omega, r->beyn_centre_im), r->beyn_rRe, r->beyn_rIm, (size_t)N);
 	double *hp = (double *)(u->pin + align256(np * 24));
-	for (int n = 0; n < N; ++n) mie_params(r, u->bcont->z_dz[n][0], hp + 9 * n);
+	double *ht = (double *)(u->pin + align256(np * 24) + align256(N * 9 * 8));
+	const complex double z0 = u->bcont->centre;
+	for (int n = 0; n < N; ++n) {
+		const complex double om = u->bcont->z_dz[n][0], dz = u->bcont->z_dz[n][1];
+		const complex double k = csqrt((complex double)r->eps_bg) * om / SPEED_OF_LIGHT;
+		mie_params(r, om, hp + 9 * n);
+		double *t = ht + 8 * n;
+		t[0] = creal(k); t[1] = cimag(k);
+		t[2] = creal(dz); t[3] = cimag(dz);
+		t[4] = creal(om - z0); t[5] = cimag(om - z0);
+		t[6] = (n % 2) == 0; t[7] = 0;
+	}
 	SRV_HIPCHK(hipMemcpyAsync(u->bpos, u->pin, np * 24, hipMemcpyHostToDevice, st));
 	SRV_HIPCHK(hipMemcpyAsync(dprm, hp, N * 9 * 8, hipMemcpyHostToDevice, st));
+	SRV_HIPCHK(hipMemcpyAsync(u->btab, ht, N * 64, hipMemcpyHostToDevice, st));
+	qh_zero(u->bcur, 256, st);
 	qh_zero(u->btt, np * 4, st);
 	qh_zero(u->dA0, L * l * 16, st);
 	qh_zero(u->dA1, L * l * 16, st);
@@ -755,30 +800,28 @@ static void launch_beyn(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int ch
 	qh_launch_expand_tdiag(lMax, N, mie, u->btd, st);
 }
 
-/* Enqueue the next chunk of quadrature nodes: M(z_i), X_i = M(z_i)^{-1} V
- * (batched LU over the chunk, refined), moments A0, A1 (+ coarse). */
-static void beyn_enqueue_chunk(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
+/* The launches of one chunk of C quadrature nodes: M(z_i), X_i = M(z_i)^{-1} V
+ * (batched LU over the chunk, refined), moments A0, A1 (+ coarse). The
+ * node-dependent launches read their node (k, dz, z - z0, coarse flag) from
+ * the node table at the device cursor, which the chunk advances by C at its
+ * end: the sequence is the same for every chunk of the contour, whether it
+ * is launched eagerly or replayed from a captured graph. */
+static void beyn_chunk_body(qpms_hip_server_t *s, qh_unit *u, hipStream_t st, int C) {
 	qh_job *jb = u->jobs[0];
 	const qpms_hip_request_t *r = &jb->req;
-	const size_t np = r->npart, L = jb->L, nb = jb->nb, l = (size_t)r->beyn_l;
-	const int N = r->beyn_nodes, n0 = u->next_node;
-	const int C = (N - n0 < u->chunk) ? N - n0 : u->chunk;
+	const size_t np = r->npart, L = jb->L, l = (size_t)r->beyn_l;
 	const qh_trans_tables *tb = qh_tables_get(srv_calc(s, r->lMax));
-	const complex double z0 = u->bcont->centre;
 	const long sM = (long)(L * L), sV = (long)(L * l);
 	double *M = u->bM, *X = u->bX, *R = u->bR;
-	for (int i = 0; i < C; ++i) {
-		const complex double om = u->bcont->z_dz[n0 + i][0];
-		const complex double k = csqrt((complex double)r->eps_bg) * om / SPEED_OF_LIGHT;
-		qh_launch_modeproblem(tb, np, u->bpos, u->btt, u->btd + (n0 + i) * nb * 2, creal(k), cimag(k), M + 2 * i * sM,
-			L, st);
-	}
+	for (int i = 0; i < C; ++i)
+		qh_launch_modeproblem_node(tb, np, u->bpos, u->btt, u->btd, u->btab, u->bcur, i, M + 2 * i * sM, L, st);
 	qh_zero(u->binfo, C * 4, st);
 	qh_zero(X, C * L * l * 16, st);
 	for (int i = 0; i < C; ++i) qh_copy_c64((long)(L * l), u->bV, R + i * sV * 2, st);
 	if (s->o.mixed) {
 		qh_d2f((long)(C * L * L), M, u->bM32, st);
-		qh_getrf_batched_ws_f32((int)L, C, u->bM32, sM, u->bipiv, u->bperm, u->binfo, u->bws, st);
+		qh_getrf_batched_wsla_f32((int)L, C, u->bM32, sM, u->bipiv, u->bperm, u->binfo, u->bws,
+			L >= SRV_LOOKAHEAD_MIN, st);
 		/* nodes near an eigenvalue make M ill-conditioned: twice the steps of a solve */
 		for (int it = 0; it < 2 * s->o.refine_steps; ++it) {
 			qh_d2f((long)(C * L * l), R, u->bR32, st);
@@ -788,15 +831,35 @@ static void beyn_enqueue_chunk(qpms_hip_server_t *s, qh_unit *u, hipStream_t st)
 			qh_gemm_sub_f64((int)L, (int)l, (int)L, C, M, (long)L, sM, X, (long)l, sV, R, (long)l, sV, st);
 		}
 	} else {
-		qh_getrf_batched_ws_f64((int)L, C, M, sM, u->bipiv, u->bperm, u->binfo, u->bws, st);
+		qh_getrf_batched_wsla_f64((int)L, C, M, sM, u->bipiv, u->bperm, u->binfo, u->bws, L >= SRV_LOOKAHEAD_MIN, st);
 		for (int i = 0; i < C; ++i) qh_copy_c64((long)(L * l), u->bV, X + i * sV * 2, st);
 		qh_getrs_batched_f64((int)L, (int)l, C, M, sM, u->bperm, X, sV, u->bW, st);
 	}
-	for (int i = 0; i < C; ++i) {
-		const int n = n0 + i;
-		const complex double dz = u->bcont->z_dz[n][1], w = u->bcont->z_dz[n][0] - z0;
-		qh_beyn_accum((long)(L * l), X + i * sV * 2, creal(dz), cimag(dz), creal(w), cimag(w), (n % 2) == 0,
-			(double *)u->dA0, (double *)u->dA1, (double *)u->dA0c, (double *)u->dA1c, st);
+	for (int i = 0; i < C; ++i)
+		qh_beyn_accum_node((long)(L * l), X + i * sV * 2, u->btab, u->bcur, i, (double *)u->dA0, (double *)u->dA1,
+			(double *)u->dA0c, (double *)u->dA1c, st);
+	qh_cursor_advance(u->bcur, C, st);
+}
+
+/* Enqueue the next chunk. With graphs on, the first chunk's launches are
+ * captured into a graph (launched at once) and every later chunk is one
+ * launch of that graph; the chunk size divides the node count, so all chunks
+ * share the sequence. */
+static void beyn_enqueue_chunk(qpms_hip_server_t *s, qh_unit *u, hipStream_t st) {
+	const int N = u->jobs[0]->req.beyn_nodes, n0 = u->next_node;
+	const int C = (N - n0 < u->chunk) ? N - n0 : u->chunk;
+	if (s->graphs && C == u->chunk && N / C > 1 && u->jobs[0]->L <= SRV_GRAPH_MAX) {
+		if (!u->gexec) {
+			hipGraph_t g;
+			SRV_HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+			beyn_chunk_body(s, u, st, C);
+			SRV_HIPCHK(hipStreamEndCapture(st, &g));
+			SRV_HIPCHK(hipGraphInstantiate(&u->gexec, g, NULL, NULL, 0));
+			SRV_HIPCHK(hipGraphDestroy(g));
+		}
+		SRV_HIPCHK(hipGraphLaunch(u->gexec, st));
+	} else {
+		beyn_chunk_body(s, u, st, C);
 	}
 	u->next_node = n0 + C;
 }
@@ -860,6 +923,8 @@ static void retire(qpms_hip_server_t *s, qh_unit *u) {
 		free(u->jobs[j]);
 	}
 	SRV_HIPCHK(hipEventDestroy(u->done));
+	if (u->gexec) SRV_HIPCHK(hipGraphExecDestroy(u->gexec));
+	u->gexec = NULL;
 	arena_free(&s->pinned, u->pin_off);
 	arena_free(&s->arena, u->arena_off);
 	s->lane_busy[u->lane] = 0;

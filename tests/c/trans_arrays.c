@@ -6,6 +6,7 @@
  *   qpms_scatsys_build_translation_matrix_full with particles of two
  *   different bases (scatsystem.c:1180-1217)
  *   qpms_scatsys_finite_find_eigenmodes             (scatsystem.c:2301-2325)
+ *   qpms_scatsys_periodic_find_eigenmodes           (scatsystem.c:2351-2370)
  *
  * usage: trans_arrays OUTDIR
  * Writes raw little-endian arrays into OUTDIR, one file per quantity (the
@@ -50,6 +51,22 @@ static qpms_vswf_set_spec_t *electric_spec(int lMax) {
 	s->n = k;
 	s->lMax_M = 0;
 	return s;
+}
+
+/* M(omega) of the 2D-periodic system at Bloch vector k: I - T(omega) W(omega, k)
+ * at the adjusted Ewald parameter, through the public builder (the callback a
+ * caller of beyn_solve writes for the periodic problem, scatsystem.c:2333-2349). */
+static int periodic_M(complex double *target, size_t m, complex double omega, void *params) {
+	const struct { const qpms_scatsys_t *ss; const double *k; } *p = params;
+	qpms_scatsys_at_omega_t *ssw = qpms_scatsys_at_omega(p->ss, omega);
+	if (m != p->ss->fecv_size) return 1;
+	qpms_scatsys_at_omega_k_t sswk;
+	sswk.ssw = ssw;
+	sswk.k[0] = p->k[0]; sswk.k[1] = p->k[1]; sswk.k[2] = p->k[2];
+	sswk.eta = qpms_ss_adjusted_eta(p->ss, ssw->wavenumber, p->k);
+	qpms_scatsyswk_build_modeproblem_matrix_full(target, &sswk);
+	qpms_scatsys_at_omega_free(ssw);
+	return 0;
 }
 
 static void dump_spec(const char *name, const qpms_vswf_set_spec_t *s) {
@@ -199,6 +216,46 @@ int main(int argc, char **argv) {
 	dump("eig_pos", bin, sizeof bin);
 	beyn_result_free(r);
 	qpms_scatsys_at_omega_free(sbw);
+
+	/* ---- 5. periodic eigenmodes: the wrapper against beyn_solve itself --- */
+	qpms_vswf_set_spec_t *bp = qpms_vswf_set_spec_from_lMax(2, QPMS_NORMALISATION_DEFAULT);
+	qpms_tmatrix_generator_sphere_param_t spp = {{qpms_epsmu_const_g, &bg}, {qpms_epsmu_const_g, &inb}, 50e-9};
+	qpms_tmatrix_generator_t gp = {qpms_tmatrix_generator_sphere, &spp};
+	qpms_tmatrix_function_t tmp = {bp, &gp};
+	qpms_ss_derived_tmatrix_t dtp;
+	memset(&dtp, 0, sizeof dtp);
+	qpms_particle_tid_t pp[2];
+	memset(pp, 0, sizeof pp);
+	pp[1].pos.x = a / sqrt(3);
+	qpms_scatsys_t sp0;
+	memset(&sp0, 0, sizeof sp0);
+	sp0.lattice_dimension = 2;
+	sp0.medium.function = qpms_epsmu_const_g;
+	sp0.medium.params = &bg;
+	sp0.tmg = &tmp; sp0.tmg_count = 1;
+	sp0.tm = &dtp; sp0.tm_count = 1;
+	sp0.p = pp; sp0.p_count = 2;
+	sp0.per.lattice_basis[0].x = b1.x; sp0.per.lattice_basis[0].y = b1.y;
+	sp0.per.lattice_basis[1].x = b2.x; sp0.per.lattice_basis[1].y = b2.y;
+	qpms_scatsys_at_omega_t *spw = qpms_scatsys_apply_symmetry(&sp0, NULL, 2.0 * EV_RAD_PER_S, NULL);
+	const double kb[3] = {0.1 * 2 * M_PI / a, 0.05 * 2 * M_PI / a, 0};
+	const complex double pc = (2.0 - 0.08 * I) * EV_RAD_PER_S;
+	const double prr = 0.25 * EV_RAD_PER_S, pri = 0.1 * EV_RAD_PER_S;
+	const size_t pm = spw->ss->fecv_size;   /* 32: l = m probe columns */
+	const double ptol = 1e-5 * 2 * M_PI * prr * (double)pm;
+	beyn_result_t *rw = qpms_scatsys_periodic_find_eigenmodes(spw->ss, kb, pc, prr, pri, 40, ptol, 1, 0.0);
+	struct per_param { const qpms_scatsys_t *ss; const double *k; } ppar = {spw->ss, kb};
+	beyn_contour_t *pcont = beyn_contour_ellipse(pc, prr, pri, 40);
+	beyn_result_t *rd = beyn_solve(pm, pm, periodic_M, NULL, &ppar, pcont, ptol, 1, 0.0);
+	const double pn[4] = {(double)rw->neig, (double)rd->neig, (double)pm, ptol};
+	dump("per_n", pn, sizeof pn);
+	dump("per_wrap_val", rw->eigval, rw->neig * sizeof(complex double));
+	dump("per_direct_val", rd->eigval, rd->neig * sizeof(complex double));
+	dump("per_wrap_sv", rw->ranktest_SV, pm * sizeof(double));
+	dump("per_direct_sv", rd->ranktest_SV, pm * sizeof(double));
+	beyn_result_free(rw);
+	beyn_result_free(rd);
+	free(pcont);
 
 	qpms_trans_calculator_free(c3);
 	qpms_trans_calculator_free(c2);

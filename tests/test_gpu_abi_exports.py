@@ -151,3 +151,18 @@ def test_finite_find_eigenmodes(out):
         assert abs(ev[j] - lam) <= 1e-9 * abs(lam), (ev[j] / EV, lam / EV)
         s = np.linalg.svd(Mfun(ev[j]), compute_uv=False)
         assert s[-1] / s[0] < 1e-8, s[-1] / s[0]
+
+
+def test_periodic_find_eigenmodes_is_beyn_on_the_mode_problem(out):
+    """qpms_scatsys_periodic_find_eigenmodes on the hexagonal two-particle
+    lattice (lMax 2, m = 32, l = m probe columns, 40-node ellipse around
+    2.0 - 0.08i eV at an in-zone Bloch vector) returns exactly what
+    beyn_solve returns for the same contour and probe seed with the periodic
+    builder I - T(omega) W(omega, k) as its callback (scatsystem.c:2327-2370):
+    the wrapper adds only the contour, the probe width and the callback."""
+    nw, nd, m, tol = out("per_n", np.float64)
+    assert m == 32 and nw == nd
+    assert np.array_equal(out("per_wrap_sv", np.float64), out("per_direct_sv", np.float64))
+    assert np.array_equal(out("per_wrap_val"), out("per_direct_val"))
+    sv = out("per_wrap_sv", np.float64)
+    assert nw == max(1, (sv > tol).sum())   # rank test of beyn.c:382 with rank_sel_min = 1

@@ -57,7 +57,7 @@ def test_w_build_2048_particles_lmax3(gpu):
     o = of.OracleCalc(lMax)
     worst = 0.0
     for i, (R, Cc) in enumerate(sorted(pairs)):
-        src = dW.ptr + 16 * (R * nb * L + Cc * nb)
+        src = int(dW.ptr + 16 * (int(R) * nb * L + int(Cc) * nb))
         assert h.hipMemcpy2D(blk.ctypes.data, 16 * nb, src, 16 * L, 16 * nb, nb, 2) == 0
         if R == Cc:
             assert not blk.any(), R

@@ -62,3 +62,31 @@ def test_gemm_sub_few_rhs_batched(gpu, N):
     out2 = d2[2].to_array(np.complex128, (5, N))
     ref2 = C2 - A2 @ B2
     assert np.max(np.abs(out2 - ref2)) / np.max(np.abs(ref2)) < 1e-13
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (32, 32, 2, 1), (64, 64, 16, 2), (70, 45, 33, 1),
+                                         (130, 129, 64, 3), (257, 300, 256, 1), (5, 200, 7, 2), (400, 17, 300, 1)])
+def test_cgemm3m_sub(gpu, tile, M, N, K, batch):
+    """FP32 complex C -= A B by the three-product (3M) MFMA kernel, every tile
+    shape, edges in M, N and K, batched with per-matrix strides and padded
+    leading dimensions."""
+    import qpms_amd
+    L = qpms_amd.lib()
+    rng = np.random.default_rng(7 * M + 11 * N + 13 * K + tile)
+    lda, ldb, ldc = K + 3, N + 5, N + 1
+    A = (rng.normal(size=(batch, M, lda)) + 1j * rng.normal(size=(batch, M, lda))).astype(np.complex64)
+    B = (rng.normal(size=(batch, K, ldb)) + 1j * rng.normal(size=(batch, K, ldb))).astype(np.complex64)
+    C = (rng.normal(size=(batch, M, ldc)) + 1j * rng.normal(size=(batch, M, ldc))).astype(np.complex64)
+    ref = C.astype(np.complex128)
+    ref[:, :, :N] -= A[:, :, :K].astype(np.complex128) @ B[:, :, :N].astype(np.complex128)
+    dA, dB, dC = (qpms_amd.DeviceBuffer.from_array(x) for x in (A, B, C))
+    f = L.qh_cgemm3m_sub
+    lg, vp, i32 = ctypes.c_long, ctypes.c_void_p, ctypes.c_int
+    f.argtypes = [i32, i32, i32, i32, vp, lg, lg, vp, lg, lg, vp, lg, lg, i32, vp]
+    f(M, N, K, batch, dA.ptr, lda, M * lda, dB.ptr, ldb, K * ldb, dC.ptr, ldc, M * ldc, tile, None)
+    L.qpms_hip_sync()
+    out = dC.to_array(np.complex64, (batch, M, ldc))
+    assert np.array_equal(out[:, :, N:], C[:, :, N:])   # columns beyond N untouched
+    err = np.max(np.abs(out - ref)) / max(1.0, np.max(np.abs(ref)))
+    assert err < 4e-6 * max(1, K), err

@@ -189,3 +189,32 @@ def test_deadline_keeps_long_requests_queued(gpu):
         srv.progress(wait=True)
     srv.close()
     assert got[rb].status == -2
+
+
+def test_beyn_chunk_graph_replay_bit_identical(gpu, monkeypatch):
+    """Beyn contours of 16 nodes in chunks of 4: the first chunk's launches are
+    captured into a graph and replayed for the other three, with plane-wave
+    solves of other requests running between the replays. The eigenvalues,
+    singular values and extinctions equal, bit for bit, those of a server that
+    launches every chunk eagerly (the trailing updates are hand-written
+    kernels at these orders in both runs)."""
+    from qpms_amd import serve as sv
+    reqs = [r for r in requests() if r.kind != sv.KIND_PERIODIC][:14]
+    assert sum(r.kind == sv.KIND_BEYN for r in reqs) == 2
+    monkeypatch.setenv("QPMS_HIP_BEYN_CHUNK", "4")
+    out = {}
+    for graphs in ("0", "1"):
+        monkeypatch.setenv("QPMS_HIP_GRAPHS", graphs)
+        srv = sv.Server(arena_gib=8, nstreams=4, mixed=True)
+        ids = [srv.submit(r) for r in reqs]
+        got = drain(srv)
+        srv.close()
+        out[graphs] = [got[i] for i in ids]
+    for r, a, b in zip(reqs, out["0"], out["1"]):
+        assert a.status == 0 and b.status == 0, (a.message, b.message)
+        if r.kind == sv.KIND_SCATTER:
+            assert a.sigma_ext == b.sigma_ext
+        else:
+            assert a.neig == b.neig and a.neig > 0
+            assert np.array_equal(np.array(a.sv[:r.c.beyn_l]), np.array(b.sv[:r.c.beyn_l]))
+            assert np.array_equal(a.eigenvalues(), b.eigenvalues())
