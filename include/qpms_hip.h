@@ -676,6 +676,19 @@ int qpms_hip_build_modeproblem_matrix_dev(const qpms_trans_calculator *c, size_t
  * device; ipiv_dev receives 1-based pivots (int). Returns 0 or the first
  * zero pivot (1-based), like zgetrf's info. */
 int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_dev, void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under graph capture, serving loops that forbid hipMalloc):
+ * qpms_hip_zgetrf_dev without any hipMalloc/hipFree or allocating library
+ * call. Its scratch (info word and pivot-tournament workspace) is a static
+ * 16 MiB device array of the library, so the order is limited to
+ * qpms_hip_zgetrf_noalloc_max_n() (127,744); a larger n returns -1
+ * with a message on stderr and touches nothing. Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other host threads, or
+ * other streams before this call's synchronisation) race. Same results as
+ * qpms_hip_zgetrf_dev up to the rounding of the trailing GEMM (hand-written
+ * MFMA kernel here). */
+int qpms_hip_zgetrf_dev_noalloc(size_t n, _Complex double *A_dev, int *ipiv_dev, void *stream);
+size_t qpms_hip_zgetrf_noalloc_max_n(void);
 /* Solve with nrhs right-hand sides stored row-major (n × nrhs). */
 int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, const int *ipiv_dev,
 		_Complex double *B_dev, void *stream);

@@ -23,6 +23,7 @@ extern "C" int qh_dev_init(void) {
     }
     HIPCHK(hipSetDevice(0));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    qh_noalloc_scratch_init();
     g_init_status = 0;
   });
   return g_init_status;

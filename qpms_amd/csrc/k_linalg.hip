@@ -1258,6 +1258,64 @@ extern "C" int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_d
   return info;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant of qpms_hip_zgetrf_dev for callers that cannot
+// allocate device memory (code under graph capture, serving loops that
+// forbid hipMalloc). Its scratch -- the info word and the tournament
+// workspace -- is the file-scope array below; the GEMMs are the
+// hand-written kernel (the library GEMM may allocate) and there is no
+// lookahead side stream. The host never names the array in a kernel
+// argument or copy: it uses the device address hipGetSymbolAddress returned
+// for the current device at initialisation.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEV = 64;
+static char *g_scratch_ptr[NOALLOC_MAX_DEV];   // device address of `scratch`, per device
+
+// Called by the library's initialisation with the device current (once per
+// device); records the device address of the scratch array.
+extern "C" void qh_noalloc_scratch_init(void) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "device %d outside the scratch table", dev);
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = (char *)p;
+}
+
+// Largest order the scratch array takes (FP64 tournament workspace after a
+// 256-byte info slot).
+extern "C" size_t qpms_hip_zgetrf_noalloc_max_n(void) {
+  size_t n = 256;
+  while (256 + PanelWS<double>::bytes((int)(n + 256)) <= NOALLOC_SCRATCH_BYTES) n += 256;
+  return n;
+}
+
+extern "C" int qpms_hip_zgetrf_dev_noalloc(size_t n, _Complex double *A_dev, int *ipiv_dev, void *stream) {
+  hipStream_t s = (hipStream_t)qh_stream(stream);
+  if (n == 0) return 0;
+  const size_t need = 256 + PanelWS<double>::bytes((int)(n < (1u << 22) ? n : (1u << 22)));
+  if (n >= (1u << 22) || need > NOALLOC_SCRATCH_BYTES) {
+    fprintf(stderr, "qpms_hip_zgetrf_dev_noalloc: order %zu needs %zu bytes of scratch, above the %zu-byte array "
+            "(largest order %zu)\n", n, need, NOALLOC_SCRATCH_BYTES, qpms_hip_zgetrf_noalloc_max_n());
+    return -1;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  char *base = (dev >= 0 && dev < NOALLOC_MAX_DEV) ? g_scratch_ptr[dev] : nullptr;
+  if (!base) qpms_hip_die(__FILE__, __LINE__, __func__, "scratch of device %d not initialised", dev);
+  int *dinfo = (int *)base;
+  qh_zero(dinfo, 256, s);   // a kernel: the info word must start at zero on every call
+  getrf_batched<double>((int)n, 1, (v2d *)A_dev, (long)n, 0, ipiv_dev, 0, nullptr, dinfo, s, base + 256,
+                        /*lookahead=*/false, /*own_gemm=*/true);
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return info;
+}
+
 extern "C" int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, const int *ipiv_dev,
                                    _Complex double *B_dev, void *stream) {
   hipStream_t s = (hipStream_t)qh_stream(stream);

@@ -46,6 +46,7 @@ int qh_gauntB_Q_max(int M, int n, int mu, int nu);
 
 /* ---- HIP launchers (k_*.hip). All take a hipStream_t as void*. ---------- */
 int qh_dev_init(void);
+void qh_noalloc_scratch_init(void);     /* k_linalg.hip: device address of the no-alloc scratch */
 int qh_zero(void *dst, size_t bytes, void *stream);   /* k_serve.hip: clear by a kernel */
 void *qh_stream(void *s);               /* NULL → library stream */
 void *qh_dmalloc(size_t bytes);
