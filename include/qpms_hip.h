@@ -681,7 +681,7 @@ int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_dev, void *s
  * qpms_hip_zgetrf_dev without any hipMalloc/hipFree or allocating library
  * call. Its scratch (info word and pivot-tournament workspace) is a static
  * 16 MiB device array of the library, so the order is limited to
- * qpms_hip_zgetrf_noalloc_max_n() (127,744); a larger n returns -1
+ * qpms_hip_zgetrf_noalloc_max_n() (125,952); a larger n returns -1
  * with a message on stderr and touches nothing. Every call on a GPU shares
  * that GPU's array: calls that overlap on one GPU (other host threads, or
  * other streams before this call's synchronisation) race. Same results as

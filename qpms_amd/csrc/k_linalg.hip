@@ -8,17 +8,17 @@
 //
 // LU structure (getrf_batched): outer blocks of LU_NB = 256 columns.
 //   k_ts_*        factor a tall strip of PW = 32 columns with tournament
-//                 pivoting: candidate rows chosen per chunk of 512 (FP32) or
-//                 256 (FP64) rows staged in LDS, merged a chunk's worth of
-//                 candidate sets at a time, the
-//                 winners moved to the top (recorded as zgetrf-style
-//                 interchanges) in every column by one gather, then L21 by a
-//                 triangular solve; no workgroup waits for another;
-//   k_trsm        U = L^{-1} B for a PW-row block (one thread per column,
-//                 the unit-lower triangle in LDS);
-//   k_gemm_sub    C -= A B: the rank-PW updates inside an outer block and
-//                 the rank-256 trailing update A22 -= L21 U12 (the O(n^3)
-//                 part). FP64 uses v_mfma_f64_16x16x4f64, FP32
+//                 pivoting: candidate rows chosen per chunk of 256 rows
+//                 staged in LDS, merged a chunk's worth of candidate sets at
+//                 a time, the winners moved to the top (recorded as
+//                 zgetrf-style interchanges) in every column by one gather;
+//                 the last level also inverts the strip's L11 and U11;
+//                 no workgroup waits for another;
+//   k_gemm_sub    C -= A B, or C = A B: L21 = A21 U11^{-1} and
+//                 U12 = L11^{-1} A12 (products with the inverses), the
+//                 rank-PW updates inside an outer block and the rank-256
+//                 trailing update A22 -= L21 U12 (the O(n^3) part, rocBLAS or
+//                 k_cgemm3m by size class). FP64 uses v_mfma_f64_16x16x4f64, FP32
 //                 v_mfma_f32_16x16x4f32; the complex product is four real
 //                 MFMA chains on de-interleaved LDS tiles.
 // Every kernel takes a batch index in blockIdx.z with per-matrix strides, so
@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <rocblas/rocblas.h>
 #include "qpms_internal.h"
@@ -65,6 +66,9 @@ template <class T> __device__ __forceinline__ typename C2<T>::t cdiv(typename C2
 
 // A12 <- L11^{-1} A12 with L11 unit lower triangular (w x w at (k0, k0)).
 // Generic form: rows r0..r0+w of B (ldb) against the triangle at L (ldl).
+// One thread per column: the column's w entries are loaded before the
+// substitution and stored after it (stores between the loads would keep
+// each row's load waiting on the previous row's store).
 template <class T, bool UNIT_LOWER, int WM = NBW>
 __global__ __launch_bounds__(256) void k_trsm(int w, int ncols, const typename C2<T>::t *L, long ldl, long sL,
                                               typename C2<T>::t *B, long ldb, long sB) {
@@ -78,26 +82,26 @@ __global__ __launch_bounds__(256) void k_trsm(int w, int ncols, const typename C
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= ncols) return;
   cT x[NBW];
+#pragma unroll
+  for (int r = 0; r < NBW; ++r) x[r] = (r < w) ? B[(long)r * ldb + c] : cT{0, 0};
   if (UNIT_LOWER) {
 #pragma unroll
-    for (int r = 0; r < NBW; ++r) {
+    for (int r = 1; r < NBW; ++r) {
       if (r < w) {
-        cT v = B[(long)r * ldb + c];
+        cT v = x[r];
 #pragma unroll
         for (int s = 0; s < r; ++s) {
           const cT l = sl[r * w + s];
           v = cT{v.x - (l.x * x[s].x - l.y * x[s].y), v.y - (l.x * x[s].y + l.y * x[s].x)};
         }
         x[r] = v;
-        B[(long)r * ldb + c] = v;
       }
     }
   } else {  // upper, non-unit: backward substitution
 #pragma unroll
-    for (int rr = NBW - 1; rr >= 0; --rr) {
-      if (rr < w) {
-        const int r = rr;
-        cT v = B[(long)r * ldb + c];
+    for (int r = NBW - 1; r >= 0; --r) {
+      if (r < w) {
+        cT v = x[r];
 #pragma unroll
         for (int s = r + 1; s < NBW; ++s) {
           if (s < w) {
@@ -105,16 +109,21 @@ __global__ __launch_bounds__(256) void k_trsm(int w, int ncols, const typename C
             v = cT{v.x - (u.x * x[s].x - u.y * x[s].y), v.y - (u.x * x[s].y + u.y * x[s].x)};
           }
         }
-        v = cdiv<T>(v, sl[r * w + r]);
-        x[r] = v;
-        B[(long)r * ldb + c] = v;
+        x[r] = cdiv<T>(v, sl[r * w + r]);
       }
     }
   }
+#pragma unroll
+  for (int r = 0; r < NBW; ++r)
+    if (r < w) B[(long)r * ldb + c] = x[r];
 }
 
 // ---------------------------------------------------------------------------
-// C[M x N] -= A[M x K] * B[K x N] (complex, row-major), MFMA.
+// C[M x N] -= A[M x K] * B[K x N] (complex, row-major), MFMA; with ASSIGN,
+// C = A B. Every workgroup reads its whole K extent before it writes its C
+// tile, so C may be A (N <= 64: one column tile) or B (M <= 64: one row
+// tile), which the strip updates use to apply 32 x 32 triangular inverses in
+// place.
 // Block: 256 threads = 4 waves, C tile 64 x 64, each wave a 32 x 32 quadrant
 // made of 2 x 2 MFMA 16x16 tiles, K step 16 (= 4 MFMA k-steps of 4).
 // LDS tiles are de-interleaved: Ar, Ai [64][16+1], Br, Bi [16][64+1].
@@ -144,7 +153,7 @@ template <> struct MF<float> {
 
 constexpr int GT = 64, GK = 16;
 
-template <class T>
+template <class T, bool ASSIGN = false>
 __global__ __launch_bounds__(256) void k_gemm_sub(int M, int N, int K, const typename C2<T>::t *A, long lda, long sA,
                                                   const typename C2<T>::t *B, long ldb, long sB,
                                                   typename C2<T>::t *C, long ldc, long sC) {
@@ -155,8 +164,11 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int M, int N, int K, const typ
   C += (long)blockIdx.z * sC;
   __shared__ T Ar[GT][GK + 1], Ai[GT][GK + 1], Br[GK][GT + 1], Bi[GK][GT + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;  // wave quadrant
+  // grid-stride over C tiles (a grid smaller than the tile count keeps its
+  // workgroups on few CUs)
+  for (int m0 = blockIdx.y * GT; m0 < M; m0 += gridDim.y * GT)
+  for (int n0 = blockIdx.x * GT; n0 < N; n0 += gridDim.x * GT) {
   acc_t pr[2][2], pi[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -205,12 +217,17 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int M, int N, int K, const typ
       for (int i = 0; i < 4; ++i) {
         const int row = m0 + wm + a * 16 + MF<T>::row(lk, i), col = n0 + wn + b * 16 + lr;
         if (row < M && col < N) {
-          cT c = C[(long)row * ldc + col];
-          c.x -= pr[a][b][i];
-          c.y -= pi[a][b][i];
-          C[(long)row * ldc + col] = c;
+          if (ASSIGN) {
+            C[(long)row * ldc + col] = cT{pr[a][b][i], pi[a][b][i]};
+          } else {
+            cT c = C[(long)row * ldc + col];
+            c.x -= pr[a][b][i];
+            c.y -= pi[a][b][i];
+            C[(long)row * ldc + col] = c;
+          }
         }
       }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,8 +369,8 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 // ---------------------------------------------------------------------------
 // Tall-strip panel: LU with tournament pivoting (CALU / TSLU, Grigori,
 // Demmel & Xiang 2011). The strip (rows c0..n-1, columns c0..c0+w-1,
-// w <= PW) is split into chunks of R rows (512 in FP32, 256 in FP64) staged
-// in LDS, one row per thread:
+// w <= PW) is split into chunks of R = 256 rows staged in LDS, one row per
+// thread:
 //   k_ts_local   each workgroup runs partial-pivoting elimination on its
 //                chunk only to choose w candidate rows, and writes their
 //                indices and original values;
@@ -362,15 +379,17 @@ __global__ void k_block_tw(int nr, int L, const v2d *T, const v2d *W, v2d *M, in
 //   the last level (local when one chunk covers the strip, else the last
 //                merge) also writes the w x w block L11\U11 as its
 //                elimination left the pivot rows, the zgetrf-style
-//                interchanges (ipiv, perm), info on a zero pivot, and the
-//                list of rows the interchanges move;
+//                interchanges (ipiv, perm), info on a zero pivot, the
+//                list of rows the interchanges move, and L11^{-1} and
+//                U11^{-1};
 //   k_ts_apply   moves those rows in every column of the matrix (one gather
 //                per column block) and stores L11\U11;
-//   k_tslu_l21   every other strip row gets L21 = A21 U11^{-1}.
+//   then every other strip row gets L21 = A21 U11^{-1} and the next PW rows
+//   of the outer block U = L11^{-1} A12, as MFMA products (k_gemm_sub).
 // No workgroup ever waits for another one; each step is its own launch.
 // ---------------------------------------------------------------------------
 constexpr int PW_ = 32;           // strip width (FP32 and FP64)
-constexpr int TS_THREADS = 512;   // threads per tournament workgroup
+constexpr int TS_THREADS = 256;   // threads per tournament workgroup (one row each)
 constexpr int TS_WAVES = TS_THREADS / 64;
 // LDS row stride (complex): conflict-free row-per-lane access with 16-byte
 // lane accesses (two FP32 complex values, or one FP64)
@@ -379,10 +398,9 @@ template <> struct TsStride<float> { static constexpr int XS = PW_ + 2; };
 template <> struct TsStride<double> { static constexpr int XS = PW_ + 1; };
 
 template <class T> struct PanelW { static constexpr int W = PW_; };
-// rows per tournament workgroup: the chunk is staged in LDS (135 KB)
-template <class T> struct TsRows;
-template <> struct TsRows<float> { static constexpr int R = 512; };
-template <> struct TsRows<double> { static constexpr int R = 256; };
+// rows per tournament workgroup, one per thread: the chunk is staged in LDS
+// (70 KB FP32, 135 KB FP64)
+template <class T> struct TsRows { static constexpr int R = TS_THREADS; };
 template <class T> constexpr int ts_fan() { return TsRows<T>::R / PW_; }   // candidate sets merged per workgroup
 template <class T> constexpr size_t ts_lds() { return (size_t)TsRows<T>::R * TsStride<T>::XS * sizeof(typename C2<T>::t); }
 
@@ -394,14 +412,23 @@ constexpr int TS_SLOTS = 8;   // strips per outer block (LU_NB / PW)
 template <class T>
 struct PanelWS {
   static size_t set_bytes() { return ((size_t)PW_ * 4 + (size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 255) / 256 * 256; }
-  static size_t fin_bytes() { return ((size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 4 * PW_ * 4 + 16 + 255) / 256 * 256; }
+  // slot: L11\U11, moved-row lists, then L11^{-1} and U11^{-1} (PW x PW each)
+  __host__ __device__ static constexpr size_t fin_inv_off() { return ((size_t)PW_ * PW_ * sizeof(typename C2<T>::t) + 4 * PW_ * 4 + 16 + 255) / 256 * 256; }
+  static size_t fin_bytes() { return fin_inv_off() + 2 * (size_t)PW_ * PW_ * sizeof(typename C2<T>::t); }
   static int sets(int n) { return (n + TsRows<T>::R - 1) / TsRows<T>::R; }
   static size_t bytes(int n) { return (2 * (size_t)sets(n) * set_bytes() + TS_SLOTS * fin_bytes() + 255) / 256 * 256; }
 };
 
+// Workgroup state of the elimination: each wave's packed winner and winning
+// row per column (two slot sets, alternating), the pivot rows as they were
+// when chosen (urow[j][j + p] = column j + p of pivot j: its U part), and
+// the pivots.
 template <class T>
 struct TsShared {
-  unsigned long long wp[2][TS_WAVES];   // each wave's packed winner
+  typedef typename C2<T>::t cT;
+  unsigned long long wp[2][TS_WAVES];
+  __align__(16) cT cand[2][TS_WAVES][PW_];
+  __align__(16) cT urow[PW_ + 1][PW_];   // one row more: pivot j's shifted row runs j positions past its own
   int sel[PW_];    // block row of pivot j
   int skey[PW_];   // key (strip row) of pivot j
   T sval[PW_];     // 1 if the pivot j is not zero, else 0
@@ -454,49 +481,6 @@ __device__ __forceinline__ unsigned long long wave_amax(unsigned long long v) {
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// Partial-pivoting elimination over the R rows of a workgroup staged in LDS
-// (X[row][XS]), used only to choose pivot rows. Row i belongs to thread i
-// (keys distinct, < 0 for an empty row). Per column: the wave arg-max of
-// |x_j|_1 on packed keys, one LDS combine of the waves (one barrier; the
-// combine slots alternate between columns),
-// then every live row is reduced against the pivot row with the pivot's
-// reciprocal. A row's owner is the only writer of it, and the pivot row is
-// not written once chosen, so the next column's barrier orders the updates.
-// The loops stay rolled: the workgroup runs them once, in lock step. On
-// return sh.sel/skey/sval hold the pivots (rows of L\U in X for the chosen
-// rows); returns their number.
-// Y[c] -= l P[c] for c in [c1, w): one complex value per step, or (FP32)
-// pairs of them as 16-byte LDS accesses.
-__device__ __forceinline__ void ts_axpy(v2f *__restrict__ Y, const v2f *__restrict__ P, v2f l, int c1, int w) {
-  int c = c1;
-  if ((c & 1) && c < w) {
-    const v2f u = P[c], y = Y[c];
-    Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
-    ++c;
-  }
-#pragma unroll 4
-  for (; c + 1 < w; c += 2) {
-    const v4f u = *reinterpret_cast<const v4f *>(P + c);
-    v4f y = *reinterpret_cast<v4f *>(Y + c);
-    y.x -= l.x * u.x - l.y * u.y;
-    y.y -= l.x * u.y + l.y * u.x;
-    y.z -= l.x * u.z - l.y * u.w;
-    y.w -= l.x * u.w + l.y * u.z;
-    *reinterpret_cast<v4f *>(Y + c) = y;
-  }
-  if (c < w) {
-    const v2f u = P[c], y = Y[c];
-    Y[c] = v2f{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
-  }
-}
-__device__ __forceinline__ void ts_axpy(v2d *__restrict__ Y, const v2d *__restrict__ P, v2d l, int c1, int w) {
-#pragma unroll 4
-  for (int c = c1; c < w; ++c) {
-    const v2d u = P[c], y = Y[c];
-    Y[c] = v2d{y.x - (l.x * u.x - l.y * u.y), y.y - (l.x * u.y + l.y * u.x)};
-  }
-}
-
 // 1 / p for the pivot as conj(p) / |p|^2 while |p|^2 is comfortably normal
 // (FP32: the hardware reciprocal, one rounding more than a division; FP64: one
 // IEEE division), instead of Smith's three dependent divisions, which remain
@@ -519,18 +503,44 @@ template <> __device__ __forceinline__ v2d ts_crcp<double>(v2d p) {
   return cdiv<double>(v2d{1, 0}, p);
 }
 
+// Partial-pivoting elimination over the R rows of a workgroup staged in LDS
+// (X[row][XS]), used only to choose pivot rows. Row i belongs to thread i
+// (keys distinct, < 0 for an empty row), which holds it in registers. Per
+// column j, a rolled loop (its body is a few KB of code, so the workgroup's
+// single pass does not wait on instruction fetch):
+//   the wave arg-max of |x_j|_1 on packed keys; each wave's winning lane
+//   writes its row and lane 0 the packed key into LDS slots that alternate
+//   between columns; one barrier; every thread combines the waves' keys and
+//   reads the pivot row from its wave's slot (one broadcast address per
+//   access); the chosen row stores itself as it is (its U part) and drops
+//   out; every live row stores its multiplier l in X[row][j] and subtracts l
+//   times the pivot row (four FMAs per complex entry).
+// The row is kept rotated, x[p] = column j + p, shifting by one per column
+// (zeros enter on the right, and stay zero: no column mask is needed).
+// A wave reaches column j + 2's slots only after every wave has passed
+// column j + 1's barrier, so two slot sets suffice. On return X[row][c] holds
+// the multipliers of each row for the columns before its selection (all
+// columns for rows not chosen), sh.urow the chosen rows' U parts and
+// sh.sel/skey/sval the pivots; returns their number.
 template <class T>
 __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsShared<T> &sh) {
   typedef typename C2<T>::t cT;
-  constexpr int R = TsRows<T>::R, XS = TsStride<T>::XS;
+  constexpr int XS = TsStride<T>::XS;
+  static_assert(TsRows<T>::R == TS_THREADS, "one row per thread");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  bool live = (t < R) && key >= 0;
+  bool live = key >= 0;
+  cT x[PW_];
+#pragma unroll
+  for (int c = 0; c < PW_; ++c) x[c] = X[t * XS + c];
   int ns = 0;
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
-    cT xj = cT{0, 0};
-    if (live) xj = X[t * XS + j];
+    const cT xj = x[0];
     const unsigned long long v = wave_amax(ts_pack<T>(live, fabs(xj.x) + fabs(xj.y), live ? key : 0, t));
+    if (t == (int)(v & 0x3FFu)) {
+#pragma unroll
+      for (int p = 0; p < PW_; ++p) sh.cand[par][wave][p] = x[p];
+    }
     if (lane == 0) sh.wp[par][wave] = v;
     __syncthreads();
     unsigned long long m = sh.wp[par][0];
@@ -539,20 +549,86 @@ __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsS
     const unsigned int mhi = (unsigned int)(m >> 32);
     if (mhi == 0) break;   // no row left (uniform)
     const int pr = (int)(m & 0x3FFu), pk = (int)(0x3FFFFFu - (unsigned)((m >> 10) & 0x3FFFFFu));
-    const bool nonzero = mhi > 1u;   // the pivot's |x_j|_1 is not zero
-    if (t == pr) live = false;
+    const bool nonzero = mhi > 1u;   // the pivot's |x_j|_1 is not zero (uniform)
     if (t == 0) { sh.sel[j] = pr; sh.skey[j] = pk; sh.sval[j] = nonzero ? T(1) : T(0); }
     ns = j + 1;
-    if (live && nonzero) {
-      const cT *P = X + pr * XS;
-      cT *Y = X + t * XS;
-      const cT l = cmul<T>(xj, ts_crcp<T>(P[j]));
-      Y[j] = l;
-      ts_axpy(Y, P, l, j + 1, w);
+    if (t == pr) {
+      live = false;
+      cT *dst = &sh.urow[j][j];
+#pragma unroll
+      for (int p = 0; p < PW_; ++p) dst[p] = x[p];
     }
+    // Every thread shifts (straight-line code, so the shift stays in place in
+    // the registers); rows that are not live, and every row at a zero pivot,
+    // shift with l = 0. A live row keeps its entry at a zero pivot.
+    const cT *P = sh.cand[par][pr >> 6];
+    cT l = cT{0, 0};
+    if (live && nonzero) l = cmul<T>(xj, ts_crcp<T>(P[0]));
+    if (live) X[t * XS + j] = nonzero ? l : xj;
+#pragma unroll
+    for (int p = 0; p + 1 < PW_; ++p) {
+      const cT u = P[p + 1], y = x[p + 1];
+      x[p] = cT{fma(-l.x, u.x, fma(l.y, u.y, y.x)), fma(-l.x, u.y, fma(-l.y, u.x, y.y))};
+    }
+    x[PW_ - 1] = cT{0, 0};
   }
   __syncthreads();
   return ns;
+}
+
+// L11^{-1} (unit lower) and U11^{-1} (upper) of a strip's PW x PW L\U into
+// out (two PW x PW blocks, row-major), for the strip's L21 and U12 updates as
+// products. v: entries e = t + TS_THREADS q of L\U. PW right-looking steps
+// over both inverses at once (forward for L, backward for U), each entry
+// owned by one thread, one barrier per step, in the LDS of X (whose rows are
+// no longer needed). A zero diagonal entry of U is taken as one: the row's
+// L21 entry is then kept, as a zero pivot leaves it.
+template <class T>
+__device__ __forceinline__ void ts_tri_inverses(typename C2<T>::t *X, const typename C2<T>::t *v, char *out) {
+  typedef typename C2<T>::t cT;
+  constexpr int NE = PW_ * PW_ / TS_THREADS, PP = PW_ * PW_;
+  const int t = threadIdx.x;
+  cT *LU = X, *Z = X + PP, *dinv = X + 3 * PP;   // Z: L^{-1}, then U^{-1} with unscaled rows
+  __syncthreads();   // every read of X's rows is done
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = t + TS_THREADS * q, i = e / PW_, c = e % PW_;
+    LU[e] = v[q];
+    Z[e] = Z[PP + e] = (i == c) ? cT{1, 0} : cT{0, 0};
+  }
+  __syncthreads();
+  if (t < PW_) {
+    const cT d = LU[t * PW_ + t];
+    dinv[t] = (d.x == 0 && d.y == 0) ? cT{1, 0} : cdiv<T>(cT{1, 0}, d);
+  }
+  __syncthreads();
+  for (int s = 0; s < PW_; ++s) {
+    const int su = PW_ - 1 - s;
+#pragma unroll
+    for (int q = 0; q < 2 * NE; ++q) {
+      const int e = t + TS_THREADS * q;
+      if (e < PP) {   // L: rows below s take row s
+        const int i = e / PW_, c = e % PW_;
+        if (i > s) {
+          const cT l = LU[i * PW_ + s], z = Z[s * PW_ + c];
+          Z[e] = cT{Z[e].x - (l.x * z.x - l.y * z.y), Z[e].y - (l.x * z.y + l.y * z.x)};
+        }
+      } else {        // U: rows above su take row su divided by its diagonal
+        const int f = e - PP, i = f / PW_, c = f % PW_;
+        if (i < su) {
+          const cT u = cmul<T>(LU[i * PW_ + su], dinv[su]), z = Z[PP + su * PW_ + c];
+          Z[e] = cT{Z[e].x - (u.x * z.x - u.y * z.y), Z[e].y - (u.x * z.y + u.y * z.x)};
+        }
+      }
+    }
+    __syncthreads();
+  }
+  cT *o = reinterpret_cast<cT *>(out);
+#pragma unroll
+  for (int q = 0; q < 2 * NE; ++q) {
+    const int e = t + TS_THREADS * q;
+    o[e] = (e < PP) ? Z[e] : cmul<T>(Z[e], dinv[(e - PP) / PW_]);
+  }
 }
 
 // Output of the last tournament level: L11\U11 rows, ipiv (1-based, matrix
@@ -561,18 +637,21 @@ __device__ __forceinline__ int ts_gepp(typename C2<T>::t *X, int key, int w, TsS
 // i <-> p is applied to it with wave ballots (a row not yet listed sits at
 // its own index).
 template <class T>
-__device__ __forceinline__ void ts_finish(const typename C2<T>::t *X, int ns, int w, int c0, TsShared<T> &sh,
+__device__ __forceinline__ void ts_finish(typename C2<T>::t *X, int ns, int w, int c0, TsShared<T> &sh,
                                           char *fin, int *ipiv, int *perm, int *info) {
   typedef typename C2<T>::t cT;
-  constexpr int XS = TsStride<T>::XS;
+  constexpr int XS = TsStride<T>::XS, NE = PW_ * PW_ / TS_THREADS;
   const int t = threadIdx.x;
   cT *lu = reinterpret_cast<cT *>(fin);
   int *lfrom = reinterpret_cast<int *>(fin + PW_ * PW_ * sizeof(cT));
   int *lto = lfrom + 2 * PW_;
   int *lcount = lto + 2 * PW_;
-  for (int e = t; e < PW_ * PW_; e += TS_THREADS) {
-    const int i = e / PW_, c = e % PW_;
-    lu[e] = (i < ns && c < w) ? X[sh.sel[i] * XS + c] : cT{0, 0};
+  cT v[NE];   // entries t + TS_THREADS q of L11\U11
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = t + TS_THREADS * q, i = e / PW_, c = e % PW_;
+    v[q] = (i < ns && c < w) ? (c >= i ? sh.urow[i][c] : X[sh.sel[i] * XS + c]) : cT{0, 0};
+    lu[e] = v[q];
   }
   if (t < 64) {
     int id = (t < w) ? t : -1, pos = id, nlist = w;
@@ -606,10 +685,11 @@ __device__ __forceinline__ void ts_finish(const typename C2<T>::t *X, int ns, in
     if (mv) { lfrom[slot] = id; lto[slot] = pos; }
     if (t == 0) *lcount = __popcll(mm);
     if (perm) {   // the moved rows as one gather: every lane loads before any stores
-      const int v = mv ? perm[c0 + id] : 0;
-      if (mv) perm[c0 + pos] = v;
+      const int pv = mv ? perm[c0 + id] : 0;
+      if (mv) perm[c0 + pos] = pv;
     }
   }
+  ts_tri_inverses<T>(X, v, fin + PanelWS<T>::fin_inv_off());
 }
 
 template <class T>
@@ -625,9 +705,19 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_local(int n, int c0, int w, c
   A += (long)z * sA;
   ws += (long)z * sW;
   const int m = n - c0, r0 = g * R, nr = min(R, m - r0);
-  for (int e = t; e < R * PW_; e += TS_THREADS) {   // coalesced along each row
-    const int r = e / PW_, c = e % PW_;
-    X[r * XS + c] = (r < nr && c < w) ? A[(long)(c0 + r0 + r) * lda + c0 + c] : cT{0, 0};
+  {   // coalesced along each row; every load in flight before the first LDS store
+    constexpr int PER = R * PW_ / TS_THREADS;
+    cT v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + TS_THREADS * q, r = e / PW_, c = e % PW_;
+      v[q] = (r < nr && c < w) ? A[(long)(c0 + r0 + r) * lda + c0 + c] : cT{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + TS_THREADS * q;
+      X[(e / PW_) * XS + e % PW_] = v[q];
+    }
   }
   __syncthreads();
   const int ns = ts_gepp<T>(X, t < nr ? r0 + t : -1, w, sh);
@@ -639,9 +729,17 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_local(int n, int c0, int w, c
   int *oidx = reinterpret_cast<int *>(set);
   cT *oval = reinterpret_cast<cT *>(set + PW_ * 4);
   if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
-  for (int e = t; e < ns * PW_; e += TS_THREADS) {   // the chosen rows' original values
-    const int i = e / PW_, c = e % PW_;
-    oval[e] = (c < w) ? A[(long)(c0 + sh.skey[i]) * lda + c0 + c] : cT{0, 0};
+  constexpr int PO = PW_ * PW_ / TS_THREADS;
+  cT v[PO];
+#pragma unroll
+  for (int q = 0; q < PO; ++q) {   // the chosen rows' original values
+    const int e = t + TS_THREADS * q, i = e / PW_, c = e % PW_;
+    v[q] = (i < ns && c < w) ? A[(long)(c0 + sh.skey[i]) * lda + c0 + c] : cT{0, 0};
+  }
+#pragma unroll
+  for (int q = 0; q < PO; ++q) {
+    const int e = t + TS_THREADS * q;
+    if (e < ns * PW_) oval[e] = v[q];
   }
 }
 
@@ -657,10 +755,20 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin,
   const int z = blockIdx.z, b = blockIdx.x, t = threadIdx.x;
   const char *in = wsin + (long)z * sW;
   const int s0 = b * F, nset = min(F, nin - s0);
-  for (int e = t; e < R * PW_; e += TS_THREADS) {
-    const int r = e / PW_, c = e % PW_, si = r / PW_, i = r % PW_;
-    X[r * XS + c] = (si < nset) ? reinterpret_cast<const cT *>(in + (long)(s0 + si) * setb + PW_ * 4)[i * PW_ + c]
-                                : cT{0, 0};
+  {   // every load in flight before the first LDS store
+    constexpr int PER = R * PW_ / TS_THREADS;
+    cT v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + TS_THREADS * q, r = e / PW_, c = e % PW_, si = r / PW_, i = r % PW_;
+      v[q] = (si < nset) ? reinterpret_cast<const cT *>(in + (long)(s0 + si) * setb + PW_ * 4)[i * PW_ + c]
+                         : cT{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + TS_THREADS * q;
+      X[(e / PW_) * XS + e % PW_] = v[q];
+    }
   }
   int key = -1;
   if (t < R && t / PW_ < nset) key = reinterpret_cast<const int *>(in + (long)(s0 + t / PW_) * setb)[t % PW_];
@@ -675,14 +783,24 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin,
   int *oidx = reinterpret_cast<int *>(out);
   cT *oval = reinterpret_cast<cT *>(out + PW_ * 4);
   if (t < PW_) oidx[t] = (t < ns) ? sh.skey[t] : -1;
-  for (int e = t; e < ns * PW_; e += TS_THREADS) {   // the chosen rows' original values, from their input set
-    const int i = e / PW_, c = e % PW_, r = sh.sel[i];
-    oval[e] = reinterpret_cast<const cT *>(in + (long)(s0 + r / PW_) * setb + PW_ * 4)[(r % PW_) * PW_ + c];
+  constexpr int PO = PW_ * PW_ / TS_THREADS;
+  cT v[PO];
+#pragma unroll
+  for (int q = 0; q < PO; ++q) {   // the chosen rows' original values, from their input set
+    const int e = t + TS_THREADS * q, i = e / PW_, c = e % PW_, r = sh.sel[i];
+    v[q] = (i < ns) ? reinterpret_cast<const cT *>(in + (long)(s0 + r / PW_) * setb + PW_ * 4)[(r % PW_) * PW_ + c]
+                    : cT{0, 0};
+  }
+#pragma unroll
+  for (int q = 0; q < PO; ++q) {
+    const int e = t + TS_THREADS * q;
+    if (e < ns * PW_) oval[e] = v[q];
   }
 }
 
 // Row interchanges of strips as gathers A[c0 + to] = A[c0 + from] over their
-// moved rows, staged in LDS per block of AP_COLS columns.
+// moved rows, per block of AP_COLS columns: every moved entry is loaded into
+// registers, then (after a barrier) stored.
 //   k_ts_apply:  one strip (its slot `fin`), columns [col_lo, col_hi), then
 //                L11\U11 into the strip's top block;
 //   k_ts_apply_defer: the strips of an outer block (slots 0..nslots-1, strip
@@ -691,7 +809,7 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin,
 constexpr int AP_COLS = 32;
 template <class T>
 __device__ __forceinline__ void ts_gather(int n, int c0, const char *fin, typename C2<T>::t *A, long lda, int j0,
-                                          typename C2<T>::t *buf, int *from, int *to, bool colok) {
+                                          int *from, int *to, bool colok) {
   typedef typename C2<T>::t cT;
   const int t = threadIdx.x;
   const int *lfrom = reinterpret_cast<const int *>(fin + PW_ * PW_ * sizeof(cT));
@@ -700,11 +818,19 @@ __device__ __forceinline__ void ts_gather(int n, int c0, const char *fin, typena
   if (t < k) { from[t] = lfrom[t]; to[t] = lto[t]; }
   __syncthreads();
   const int c = t % AP_COLS, r4 = t / AP_COLS;
-  for (int e = r4; e < k; e += 256 / AP_COLS)
-    if (colok) buf[e * AP_COLS + c] = A[(long)(c0 + from[e]) * lda + j0 + c];
+  constexpr int RP = 256 / AP_COLS, NQ = 2 * PW_ / RP;   // rows per pass; passes over the longest list
+  cT v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {   // every load in flight, then (after the barrier) every store
+    const int e = r4 + RP * q;
+    v[q] = (colok && e < k) ? A[(long)(c0 + from[e]) * lda + j0 + c] : cT{0, 0};
+  }
   __syncthreads();
-  for (int e = r4; e < k; e += 256 / AP_COLS)
-    if (colok) A[(long)(c0 + to[e]) * lda + j0 + c] = buf[e * AP_COLS + c];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = r4 + RP * q;
+    if (colok && e < k) A[(long)(c0 + to[e]) * lda + j0 + c] = v[q];
+  }
   __syncthreads();
 }
 
@@ -712,14 +838,13 @@ template <class T>
 __global__ __launch_bounds__(256) void k_ts_apply(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA,
                                                   const char *fin, long sW, int col_lo, int col_hi) {
   typedef typename C2<T>::t cT;
-  __shared__ cT buf[2 * PW_ * AP_COLS];
   __shared__ int from[2 * PW_], to[2 * PW_];
   const int z = blockIdx.z, t = threadIdx.x, j0 = col_lo + blockIdx.x * AP_COLS;
   A += (long)z * sA;
   fin += (long)z * sW;
   const int c = t % AP_COLS, r4 = t / AP_COLS;
   const bool colok = j0 + c < col_hi;
-  ts_gather<T>(n, c0, fin, A, lda, j0, buf, from, to, colok);
+  ts_gather<T>(n, c0, fin, A, lda, j0, from, to, colok);
   const cT *lu = reinterpret_cast<const cT *>(fin);
   if (colok && j0 + c >= c0 && j0 + c < c0 + w)
     for (int i = r4; i < w; i += 256 / AP_COLS) A[(long)(c0 + i) * lda + j0 + c] = lu[i * PW_ + (j0 + c - c0)];
@@ -729,8 +854,6 @@ template <class T>
 __global__ __launch_bounds__(256) void k_ts_apply_defer(int n, int K0, int nslots, typename C2<T>::t *A, long lda,
                                                         long sA, const char *fin0, long finb, long sW, int skip_lo,
                                                         int skip_hi) {
-  typedef typename C2<T>::t cT;
-  __shared__ cT buf[2 * PW_ * AP_COLS];
   __shared__ int from[2 * PW_], to[2 * PW_];
   const int z = blockIdx.z, j0 = blockIdx.x * AP_COLS;
   if (j0 >= skip_lo && j0 + AP_COLS <= skip_hi) return;   // whole block inside the skipped columns (uniform)
@@ -738,40 +861,7 @@ __global__ __launch_bounds__(256) void k_ts_apply_defer(int n, int K0, int nslot
   fin0 += (long)z * sW;
   const int c = threadIdx.x % AP_COLS;
   const bool colok = j0 + c < n && (j0 + c < skip_lo || j0 + c >= skip_hi);
-  for (int q = 0; q < nslots; ++q) ts_gather<T>(n, K0 + PW_ * q, fin0 + q * finb, A, lda, j0, buf, from, to, colok);
-}
-
-// L21 = A21 U11^{-1}: one thread per strip row below the top block.
-template <class T>
-__global__ __launch_bounds__(256) void k_tslu_l21(int n, int c0, int w, typename C2<T>::t *A, long lda, long sA) {
-  typedef typename C2<T>::t cT;
-  constexpr int PW = PanelW<T>::W;
-  __shared__ cT U[PW * PW];
-  __shared__ cT Dinv[PW];   // reciprocals of U's diagonal (1 where it is zero: the row entry is kept)
-  A += (long)blockIdx.z * sA;
-  for (int e = threadIdx.x; e < w * w; e += 256) U[(e / w) * PW + e % w] = A[(long)(c0 + e / w) * lda + c0 + e % w];
-  if (threadIdx.x < w) {
-    const cT d = A[(long)(c0 + threadIdx.x) * lda + c0 + threadIdx.x];
-    Dinv[threadIdx.x] = (d.x == 0 && d.y == 0) ? cT{1, 0} : cdiv<T>(cT{1, 0}, d);
-  }
-  __syncthreads();
-  const int r = c0 + w + blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  cT *row = A + (long)r * lda + c0;
-  cT x[PW];
-#pragma unroll
-  for (int i = 0; i < PW; ++i) {
-    if (i < w) {
-      cT v = row[i];
-#pragma unroll
-      for (int k = 0; k < i; ++k) {
-        const cT u = U[k * PW + i];
-        v = cT{v.x - (x[k].x * u.x - x[k].y * u.y), v.y - (x[k].x * u.y + x[k].y * u.x)};
-      }
-      x[i] = cmul<T>(v, Dinv[i]);
-      row[i] = x[i];
-    }
-  }
+  for (int q = 0; q < nslots; ++q) ts_gather<T>(n, K0 + PW_ * q, fin0 + q * finb, A, lda, j0, from, to, colok);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,9 +885,11 @@ static bool use_rocblas() {
 // stream (the serving loop's lanes) must call it first, so that a later
 // stream created at the same address never inherits a dead stream's state.
 struct LaCtx {
-  hipStream_t side;
-  hipEvent_t ready, panel;
+  hipStream_t side[2];   // the trailing update beyond the next block, per order class (below / from LA_BIG)
+  bool made[2], masked[2];
+  hipEvent_t go, done;
 };
+constexpr int LA_BIG = 24576;   // orders from here on keep fewer CUs free (their GEMM is the longer part)
 constexpr int STREAM_TAB = 64;
 struct StreamState {
   hipStream_t key;
@@ -833,17 +925,48 @@ static rocblas_handle blas_for(hipStream_t s) {
   return e->blas;
 }
 
-// Lookahead context of a caller stream: the side stream that factors the
-// next block's strips and the two events ordering it against the caller's
-// stream (default priority; created once per caller stream).
-static LaCtx *la_for(hipStream_t s) {
+// CUs kept free of the side stream's trailing update, so that the next
+// block's strips (latency-bound launches of tens of workgroups, launched as
+// whole-CU workgroups) find CUs while it runs: QPMS_HIP_LU_RESERVE for orders
+// below LA_BIG (default 64), QPMS_HIP_LU_RESERVE_BIG from it on (default 32);
+// 0: no CU mask (the strips then wait for CUs to drain).
+static int lu_reserve_cus(bool big) {
+  static int v[2] = {-1, -1};
+  if (v[big] < 0) {
+    const char *e = getenv(big ? "QPMS_HIP_LU_RESERVE_BIG" : "QPMS_HIP_LU_RESERVE");
+    v[big] = e ? std::max(0, atoi(e)) : (big ? 32 : 64);
+  }
+  return v[big];
+}
+
+// Lookahead context of a caller stream: per order class a side stream,
+// masked to all CUs but the reserved ones, that runs the bulk of each
+// trailing update while the caller's stream factors the next block's strips,
+// and the two events ordering them (default priority; created on first use).
+static LaCtx *la_for(hipStream_t s, bool big) {
   std::lock_guard<std::mutex> g(g_tab_mu);
   StreamState *e = state_for(s);
   if (!e->la) {
-    HIPCHK(hipStreamCreateWithFlags(&e->lc.side, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&e->lc.ready, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&e->lc.panel, hipEventDisableTiming));
+    e->lc.made[0] = e->lc.made[1] = false;
+    HIPCHK(hipEventCreateWithFlags(&e->lc.go, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->lc.done, hipEventDisableTiming));
     e->la = true;
+  }
+  if (!e->lc.made[big]) {
+    const int reserve = lu_reserve_cus(big);
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (reserve > 0 && ncu > 2 * reserve && ncu <= 32 * 32) {
+      uint32_t mask[32] = {0};
+      for (int i = 0; i < ncu - reserve; ++i) mask[i / 32] |= 1u << (i % 32);
+      HIPCHK(hipExtStreamCreateWithCUMask(&e->lc.side[big], (uint32_t)((ncu + 31) / 32), mask));
+      e->lc.masked[big] = true;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&e->lc.side[big], hipStreamNonBlocking));
+      e->lc.masked[big] = false;
+    }
+    e->lc.made[big] = true;
   }
   return &e->lc;
 }
@@ -856,14 +979,28 @@ extern "C" void qh_stream_release(void *stream) {
   for (int i = 0; i < g_ntab; ++i) {
     if (g_tab[i].key != s) continue;
     StreamState &e = g_tab[i];
+    hipStream_t side[2] = {nullptr, nullptr};
     if (e.la) {
-      HIPCHK(hipStreamSynchronize(e.lc.side));
-      HIPCHK(hipEventDestroy(e.lc.ready));
-      HIPCHK(hipEventDestroy(e.lc.panel));
-      HIPCHK(hipStreamDestroy(e.lc.side));
+      for (int c = 0; c < 2; ++c)
+        if (e.lc.made[c]) {
+          side[c] = e.lc.side[c];
+          HIPCHK(hipStreamSynchronize(side[c]));
+        }
+      HIPCHK(hipEventDestroy(e.lc.go));
+      HIPCHK(hipEventDestroy(e.lc.done));
     }
     if (e.blas) rocblas_destroy_handle(e.blas);
     g_tab[i] = g_tab[--g_ntab];
+    for (int c = 0; c < 2; ++c) {   // each side stream's own state (its rocBLAS handle), then the stream
+      if (!side[c]) continue;
+      for (int k = 0; k < g_ntab; ++k) {
+        if (g_tab[k].key != side[c]) continue;
+        if (g_tab[k].blas) rocblas_destroy_handle(g_tab[k].blas);
+        g_tab[k] = g_tab[--g_ntab];
+        break;
+      }
+      HIPCHK(hipStreamDestroy(side[c]));
+    }
     return;
   }
 }
@@ -932,6 +1069,39 @@ static bool use_lookahead() {
   return v == 1;
 }
 
+// Strips that run beside the side stream's trailing update launch every
+// kernel with a dynamic-LDS request that fills the rest of a CU's 160 KB, so
+// that none of their workgroups shares a CU with a GEMM workgroup (they would
+// get a small share of its issue slots); they then run on the CUs the side
+// stream's mask leaves free. The request per kernel is set once with its
+// dynamic-LDS limit, from the kernel's static LDS.
+constexpr size_t CU_LDS = (size_t)160 << 10;
+static size_t whole_cu_lds(const void *fn) {
+  hipFuncAttributes at;
+  HIPCHK(hipFuncGetAttributes(&at, fn));
+  const size_t stat = at.sharedSizeBytes;
+  const size_t pad = (CU_LDS > stat + 512 ? CU_LDS - stat - 512 : 0) / 256 * 256;
+  if (pad > 0) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pad));
+  return pad;
+}
+template <class T>
+struct WholeCu {   // dynamic-LDS requests of the strip kernels for whole-CU launches
+  size_t local, merge, apply, gemm_assign, gemm_sub;
+};
+template <class T>
+static const WholeCu<T> &whole_cu() {
+  static WholeCu<T> w;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    w.local = std::max(whole_cu_lds((const void *)k_ts_local<T>), ts_lds<T>());
+    w.merge = std::max(whole_cu_lds((const void *)k_ts_merge<T>), ts_lds<T>());
+    w.apply = whole_cu_lds((const void *)k_ts_apply<T>);
+    w.gemm_assign = whole_cu_lds((const void *)k_gemm_sub<T, true>);
+    w.gemm_sub = whole_cu_lds((const void *)k_gemm_sub<T, false>);
+  });
+  return w;
+}
+
 // The tournament kernels stage a chunk of more than 64 KB of LDS: raise
 // their dynamic-LDS limit once (before any launch sequence is captured).
 template <class T>
@@ -941,6 +1111,7 @@ static void ts_prepare() {
     HIPCHK(hipFuncSetAttribute((const void *)k_ts_local<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
     HIPCHK(hipFuncSetAttribute((const void *)k_ts_merge<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ts_lds<T>()));
   });
+  (void)whole_cu<T>();   // raises the limits further for whole-CU launches
 }
 
 extern "C" void qh_cgemm3m_prepare(void);
@@ -961,8 +1132,9 @@ static size_t getrf_ws_bytes(int n, int batch) {
 // Blocked right-looking LU with tournament-pivoted panels of `batch` n x n
 // matrices (stride sA), in place. Outer blocks of LU_NB columns; inside a
 // block the columns are factorised PW at a time (k_ts_*), the block's
-// remaining columns updated by TRSM + GEMM (K = PW); then U12 = L11^{-1} A12
-// blocked by PW and the trailing A22 -= L21 U12 (K = LU_NB).
+// remaining columns updated by products with the strip's L11^{-1} and a GEMM
+// (K = PW); then U12 = L11^{-1} A12 by PW rows the same way, and the trailing
+// A22 -= L21 U12 (K = LU_NB).
 // ipiv (1-based, stride sP) lists the row interchanges as zgetrf's does;
 // perm (optional) tracks P; info[z] gets the first zero pivot. ws:
 // getrf_ws_bytes<T>(n, batch) bytes, or nullptr for a library-owned buffer
@@ -971,6 +1143,7 @@ template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
                           int *perm, int *dinfo, hipStream_t s, char *ws = nullptr, bool lookahead = true,
                           bool own_gemm = false) {
+  typedef typename C2<T>::t cT;
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
   if (n <= 0 || batch <= 0) return;
@@ -978,6 +1151,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   const int Gmax = PanelWS<T>::sets(n);
   const long sW = (long)PanelWS<T>::bytes(n);
   const long setb = (long)PanelWS<T>::set_bytes();
+  const long sWc = sW / (long)sizeof(cT);   // the workspace stride in complex entries
   if (!ws) {
     static char *own = nullptr;
     static size_t own_bytes = 0;
@@ -997,84 +1171,91 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // The strips of the outer block [k0, k0 + nb): tournament, interchanges on
   // the block's own columns, L21, then the block's remaining columns by TRSM
   // and the rank-w update. Touches only the block's columns (rows >= k0).
-  auto block_strips = [&](int k0, int nb, hipStream_t st) {
+  auto block_strips = [&](int k0, int nb, hipStream_t st, bool wcu) {
     constexpr int F = ts_fan<T>();
+    const WholeCu<T> &W = whole_cu<T>();
+    const size_t l_local = wcu ? W.local : ts_lds<T>(), l_merge = wcu ? W.merge : ts_lds<T>();
+    const size_t l_apply = wcu ? W.apply : 0, l_asg = wcu ? W.gemm_assign : 0, l_sub = wcu ? W.gemm_sub : 0;
     for (int j0 = 0; j0 < nb; j0 += PW) {
       const int w = min(PW, nb - j0), c0 = k0 + j0;
       char *fin = fin0 + (long)(j0 / PW) * finb;
       const int G = (n - c0 + TsRows<T>::R - 1) / TsRows<T>::R;
-      QH_LAUNCH(k_ts_local<T>, dim3(G, 1, batch), bts, ts_lds<T>(), st, n, c0, w, A, lda, sA, ws, sW, setb,
+      QH_LAUNCH(k_ts_local<T>, dim3(G, 1, batch), bts, l_local, st, n, c0, w, A, lda, sA, ws, sW, setb,
                          (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
       char *cur = bufA, *nxt = bufB;
       for (int nsets = G; nsets > 1; nsets = (nsets + F - 1) / F) {
         const int nout = (nsets + F - 1) / F;
-        QH_LAUNCH(k_ts_merge<T>, dim3(nout, 1, batch), bts, ts_lds<T>(), st, c0, w, nsets, cur, nxt, sW,
+        QH_LAUNCH(k_ts_merge<T>, dim3(nout, 1, batch), bts, l_merge, st, c0, w, nsets, cur, nxt, sW,
                            setb, fin, nout == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
         char *tmp = cur; cur = nxt; nxt = tmp;
       }
-      QH_LAUNCH(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, st, n, c0, w, A, lda,
-                         sA, fin, sW, k0, k0 + nb);
-      if (n - c0 - w > 0)
-        QH_LAUNCH(k_tslu_l21<T>, dim3((n - c0 - w + 255) / 256, 1, batch), b256, 0, st, n, c0, w, A, lda, sA);
+      QH_LAUNCH(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, l_apply, st, n, c0, w, A,
+                lda, sA, fin, sW, k0, k0 + nb);
+      const cT *linv = reinterpret_cast<const cT *>(fin + PanelWS<T>::fin_inv_off()), *uinv = linv + PW * PW;
+      const int mr = n - c0 - w;
+      if (mr > 0)   // L21 = A21 U11^{-1}
+        QH_LAUNCH((k_gemm_sub<T, true>), dim3(1, (mr + GT - 1) / GT, batch), b256, l_asg, st, mr, w, w,
+                  A + (long)(c0 + w) * lda + c0, lda, sA, uinv, (long)PW, sWc, A + (long)(c0 + w) * lda + c0, lda,
+                  sA);
       const int rest = k0 + nb - (c0 + w);
-      if (rest > 0) {
-        QH_LAUNCH((k_trsm<T, true, PW>), dim3((rest + 255) / 256, 1, batch), b256, 0, st, w, rest,
-                           A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);
-        const int mr = n - c0 - w;
+      if (rest > 0) {   // the block's next columns: U = L11^{-1} A12, then A22 -= L21 U
+        QH_LAUNCH((k_gemm_sub<T, true>), dim3((rest + GT - 1) / GT, 1, batch), b256, l_asg, st, w, rest, w, linv,
+                  (long)PW, sWc, A + (long)c0 * lda + c0 + w, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);
         if (mr > 0)
-          QH_LAUNCH(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, 0, st, mr,
+          QH_LAUNCH(k_gemm_sub<T>, dim3((rest + GT - 1) / GT, (mr + GT - 1) / GT, batch), b256, l_sub, st, mr,
                              rest, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA,
                              A + (long)(c0 + w) * lda + c0 + w, lda, sA);
       }
     }
   };
-  // Lookahead: block k+1's strips run on a side stream once the trailing
-  // update has reached block k+1's columns, while the rest of that update
-  // runs on `s`. The interchanges of a block are applied to the columns
-  // outside it afterwards, on `s` (the strips touch only their own block).
+  // Lookahead: once the trailing update has reached block k+1's columns, the
+  // rest of it runs on the side stream (masked off a few CUs) while block
+  // k+1's strips run here; the side stream's update is enqueued first, so it
+  // starts at once, and the strips find the reserved CUs free. Block k+1's
+  // interchanges are applied to the columns outside it afterwards (the strips
+  // touch only their own block), after the side stream's update.
   const bool la = lookahead && use_lookahead() && n > 2 * LU_NB && !stream_capturing(s);
-  LaCtx *cx = la ? la_for(s) : nullptr;
-  hipStream_t sp = la ? cx->side : s;
+  const bool big = n >= LA_BIG;
+  LaCtx *cx = la ? la_for(s, big) : nullptr;
   if (perm) QH_LAUNCH(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
-  if (la) {
-    HIPCHK(hipEventRecord(cx->ready, s));
-    HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
-  }
-  block_strips(0, min(LU_NB, n), sp);
-  if (la) HIPCHK(hipEventRecord(cx->panel, sp));
+  block_strips(0, min(LU_NB, n), s, false);
   for (int k0 = 0; k0 < n; k0 += LU_NB) {
     const int nb = min(LU_NB, n - k0);
-    if (la) HIPCHK(hipStreamWaitEvent(s, cx->panel, 0));
     if (n > nb)
       QH_LAUNCH(k_ts_apply_defer<T>, dim3((n + AP_COLS - 1) / AP_COLS, 1, batch), b256, 0, s, n, k0,
                          (nb + PW - 1) / PW, A, lda, sA, fin0, finb, sW, k0, k0 + nb);
     const int nr = n - k0 - nb;
     if (nr > 0) {
-      // U12 = L11^{-1} A12, PW rows at a time
+      // U12 = L11^{-1} A12, PW rows at a time (each strip's L^{-1} from its slot)
       for (int j0 = 0; j0 < nb; j0 += PW) {
         const int w = min(PW, nb - j0), c0 = k0 + j0;
-        QH_LAUNCH((k_trsm<T, true, PW>), dim3((nr + 255) / 256, 1, batch), b256, 0, s, w, nr,
-                           A + (long)c0 * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA);
+        const cT *linv = reinterpret_cast<const cT *>(fin0 + (long)(j0 / PW) * finb + PanelWS<T>::fin_inv_off());
+        QH_LAUNCH((k_gemm_sub<T, true>), dim3((nr + GT - 1) / GT, 1, batch), b256, 0, s, w, nr, w, linv, (long)PW,
+                  sWc, A + (long)c0 * lda + k0 + nb, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA);
         const int below = nb - j0 - w;
         if (below > 0)
           QH_LAUNCH(k_gemm_sub<T>, dim3((nr + GT - 1) / GT, (below + GT - 1) / GT, batch), b256, 0, s, below,
                              nr, w, A + (long)(c0 + w) * lda + c0, lda, sA, A + (long)c0 * lda + k0 + nb, lda, sA,
                              A + (long)(c0 + w) * lda + k0 + nb, lda, sA);
       }
-      // A22 -= L21 U12: the next block's columns first, then its strips, then the rest
+      // A22 -= L21 U12: the next block's columns first, then the rest and the next block's strips
       const int nb1 = min(LU_NB, nr);
       trailing_gemm<T>(nr, nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA, A + (long)k0 * lda + k0 + nb, lda,
                        sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s, own_gemm);
-      if (la) {
-        HIPCHK(hipEventRecord(cx->ready, s));
-        HIPCHK(hipStreamWaitEvent(sp, cx->ready, 0));
-      }
-      block_strips(k0 + nb, nb1, sp);
-      if (la) HIPCHK(hipEventRecord(cx->panel, sp));
-      if (nr > nb1)
+      const bool split = la && nr > nb1;
+      if (nr > nb1) {
+        hipStream_t sg = split ? cx->side[big] : s;
+        if (split) {
+          HIPCHK(hipEventRecord(cx->go, s));
+          HIPCHK(hipStreamWaitEvent(sg, cx->go, 0));
+        }
         trailing_gemm<T>(nr, nr - nb1, nb, batch, A + (long)(k0 + nb) * lda + k0, lda, sA,
                          A + (long)k0 * lda + k0 + nb + nb1, lda, sA, A + (long)(k0 + nb) * lda + k0 + nb + nb1, lda,
-                         sA, s, own_gemm);
+                         sA, sg, own_gemm);
+        if (split) HIPCHK(hipEventRecord(cx->done, sg));
+      }
+      block_strips(k0 + nb, nb1, s, split && cx->masked[big]);
+      if (split) HIPCHK(hipStreamWaitEvent(s, cx->done, 0));
     }
   }
   HIPCHK(hipGetLastError());
@@ -1364,9 +1545,10 @@ extern "C" int qh_getrf_batched_ws_f32(int n, int batch, float *A, long sA, int 
                        false);
   return 0;
 }
-// The same with the outer blocks' strips on a lookahead side stream of
-// `stream` when `lookahead` (large orders: the strip chain then overlaps the
-// trailing GEMM; the side stream belongs to `stream` until released).
+// The same with lookahead when `lookahead` (large orders): the bulk of each
+// trailing GEMM runs on a side stream of `stream` while the next block's
+// strips run on `stream` (the side stream belongs to `stream` until
+// released).
 extern "C" int qh_getrf_batched_wsla_f32(int n, int batch, float *A, long sA, int *ipiv, int *perm, int *info,
                                          void *ws, int lookahead, void *stream) {
   getrf_batched<float>(n, batch, (v2f *)A, n, sA, ipiv, n, perm, info, (hipStream_t)qh_stream(stream), (char *)ws,
