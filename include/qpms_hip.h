@@ -674,12 +674,14 @@ int qpms_hip_build_modeproblem_matrix_dev(const qpms_trans_calculator *c, size_t
 
 /* Dense FP64 LU with partial pivoting, row-major n × n, in place on the
  * device; ipiv_dev receives 1-based pivots (int). Returns 0 or the first
- * zero pivot (1-based), like zgetrf's info. */
+ * zero pivot (1-based), like zgetrf's info. The pivots are zgetrf's: pivot j
+ * is the first row of largest |re| + |im| in column j at or below the
+ * diagonal (replaces LAPACKE_zgetrf at scatsystem.c:2214). */
 int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_dev, void *stream);
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory (code under graph capture, serving loops that forbid hipMalloc):
  * qpms_hip_zgetrf_dev without any hipMalloc/hipFree or allocating library
- * call. Its scratch (info word and pivot-tournament workspace) is a static
+ * call. Its scratch (info word and strip workspace) is a static
  * 16 MiB device array of the library, so the order is limited to
  * qpms_hip_zgetrf_noalloc_max_n() (125,952); a larger n returns -1
  * with a message on stderr and touches nothing. Every call on a GPU shares

@@ -798,6 +798,172 @@ __global__ __launch_bounds__(TS_THREADS) void k_ts_merge(int c0, int w, int nin,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Partial-pivoting strips, for the entry points that return the factors
+// (qpms_hip_zgetrf_dev and the ABI factorisation built on it): the pivots of
+// zgetrf (zgetf2's column loop, scatsystem.c:2214): pivot j is the first row
+// of largest |re| + |im| in column j at or below the diagonal, taken in full
+// double (or float) precision. One launch per column (k_pp_step): every
+// workgroup applies the previous column's interchange and elimination to its
+// rows of the strip (the pivot row and the old top row come from copies in the
+// workspace, so no workgroup reads a row another one rewrites), then reduces
+// the next column's candidates; the last workgroup to finish (an atomic
+// count after a device-scope release) combines the partial maxima, records
+// the pivot and copies the pivot row and the top row for the next launch.
+// k_pp_finish then writes the strip's slot (L11\U11, the moved-row list,
+// L11^{-1}, U11^{-1}) as the tournament's last level does, so the rest of the
+// LU is shared. L21 is left in place by the elimination.
+// ---------------------------------------------------------------------------
+constexpr int PP_ROWS = 64;   // rows per workgroup: 8 passes of 8 rows x 32 columns
+// per-matrix workspace layout (in the tournament's candidate area, unused here)
+constexpr long PP_CNT = 0, PP_PROW = 256, PP_TROW = 768, PP_PIV = 1280, PP_NZ = 1408, PP_PART = 1536;
+__host__ __device__ constexpr long pp_part_row_off(int G) { return PP_PART + ((long)G * 8 + 255) / 256 * 256; }
+__host__ __device__ constexpr long pp_ws_bytes(int n) {
+  return pp_part_row_off((n + PP_ROWS - 1) / PP_ROWS) + ((long)((n + PP_ROWS - 1) / PP_ROWS) * 4 + 255) / 256 * 256;
+}
+
+__global__ void k_pp_init(char *ws, long sW) { *reinterpret_cast<int *>(ws + (long)blockIdx.x * sW + PP_CNT) = 0; }
+
+// (magnitude, row) pairs: larger magnitude first, then the smaller row
+__device__ __forceinline__ void pp_better(double &m, int &r, double m2, int r2) {
+  if (m2 > m || (m2 == m && r2 < r)) { m = m2; r = r2; }
+}
+__device__ __forceinline__ void pp_reduce_wg(double &m, int &r, double *sm, int *sr) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double m2 = __shfl_xor(m, off);
+    const int r2 = __shfl_xor(r, off);
+    pp_better(m, r, m2, r2);
+  }
+  const int t = threadIdx.x, wave = t >> 6;
+  if ((t & 63) == 0) { sm[wave] = m; sr[wave] = r; }
+  __syncthreads();
+  m = sm[0];
+  r = sr[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) pp_better(m, r, sm[q], sr[q]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pp_step(int n, int c0, int w, int j, typename C2<T>::t *A, long lda, long sA,
+                                                 char *ws, long sW, int *ipiv, long sP, int *info) {
+  typedef typename C2<T>::t cT;
+  const int z = blockIdx.z, g = blockIdx.x, G = gridDim.x, t = threadIdx.x, k = t & 31, rr = t >> 5;
+  A += (long)z * sA;
+  char *wz = ws + (long)z * sW;
+  ipiv += (long)z * sP;
+  info += z;
+  cT *prow = reinterpret_cast<cT *>(wz + PP_PROW), *trow = reinterpret_cast<cT *>(wz + PP_TROW);
+  int *piv = reinterpret_cast<int *>(wz + PP_PIV), *nz = reinterpret_cast<int *>(wz + PP_NZ);
+  double *part_m = reinterpret_cast<double *>(wz + PP_PART);
+  int *part_r = reinterpret_cast<int *>(wz + pp_part_row_off(G));
+  int *cnt = reinterpret_cast<int *>(wz + PP_CNT);
+  const int s = j - 1, r = c0 + s;
+  int p = -1;
+  bool pnz = false;
+  cT pk = cT{0, 0}, tk = cT{0, 0}, rcp = cT{0, 0};
+  if (j > 0) {
+    p = piv[s];
+    pnz = nz[s] != 0;
+    if (k < w) { pk = prow[k]; tk = trow[k]; }
+    if (pnz) rcp = cdiv<T>(cT{1, 0}, prow[s]);
+  }
+  double best = -1.0;
+  int bestrow = 0x7fffffff;
+  for (int q = 0; q < PP_ROWS / 8; ++q) {
+    const int i = c0 + g * PP_ROWS + q * 8 + rr;   // uniform per 32-lane group
+    const bool inrow = i < n;
+    cT x = (inrow && k < w) ? A[(long)i * lda + c0 + k] : cT{0, 0};
+    if (j > 0) {
+      if (i == r) x = pk;            // the pivot row moves to the top
+      else if (i == p) x = tk;       // the old top row takes its place
+      const T xs_re = __shfl(x.x, s, 32), xs_im = __shfl(x.y, s, 32);
+      if (i > r && pnz) {
+        const cT l = cmul<T>(cT{xs_re, xs_im}, rcp);
+        if (k == s) x = l;
+        else if (k > s) x = cT{x.x - (l.x * pk.x - l.y * pk.y), x.y - (l.x * pk.y + l.y * pk.x)};
+      }
+      if (inrow && k < w && i >= r) A[(long)i * lda + c0 + k] = x;
+    }
+    if (j < w && inrow && k == j && i >= c0 + j) pp_better(best, bestrow, (double)(fabs(x.x) + fabs(x.y)), i);
+  }
+  __shared__ double sm[4];
+  __shared__ int sr[4];
+  __shared__ int last;
+  pp_reduce_wg(best, bestrow, sm, sr);
+  if (t == 0) { part_m[g] = best; part_r[g] = bestrow; }
+  __threadfence();   // release: this workgroup's rows and partial maximum
+  if (t == 0) last = (atomicAdd(cnt, 1) == G - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();   // acquire: every other workgroup's rows and partial maxima
+  if (j < w) {
+    best = -1.0;
+    bestrow = 0x7fffffff;
+    for (int q = t; q < G; q += blockDim.x) pp_better(best, bestrow, part_m[q], part_r[q]);
+    __syncthreads();   // sm/sr reused
+    pp_reduce_wg(best, bestrow, sm, sr);
+    if (bestrow < c0 + j || bestrow >= n) { bestrow = c0 + j; best = 0.0; }   // no comparable candidate (NaN column)
+    if (t == 0) {
+      piv[j] = bestrow;
+      nz[j] = best > 0.0;
+      ipiv[c0 + j] = bestrow + 1;
+      if (!(best > 0.0) && *info == 0) *info = c0 + j + 1;
+    }
+    if (t < w) prow[t] = A[(long)bestrow * lda + c0 + t];
+    else if (t >= 32 && t - 32 < w) trow[t - 32] = A[(long)(c0 + j) * lda + c0 + t - 32];
+  }
+  if (t == 0) *cnt = 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pp_finish(int n, int c0, int w, const typename C2<T>::t *A, long lda,
+                                                   long sA, char *ws, long sW, long finoff, int *perm, long sP) {
+  typedef typename C2<T>::t cT;
+  constexpr int NE = PW_ * PW_ / 256;
+  extern __shared__ __align__(16) char smem_raw[];
+  cT *X = reinterpret_cast<cT *>(smem_raw);
+  const int z = blockIdx.z, t = threadIdx.x;
+  A += (long)z * sA;
+  char *wz = ws + (long)z * sW;
+  char *fin = wz + finoff;
+  const int *piv = reinterpret_cast<const int *>(wz + PP_PIV);
+  cT *lu = reinterpret_cast<cT *>(fin);
+  int *lfrom = reinterpret_cast<int *>(fin + PW_ * PW_ * sizeof(cT));
+  int *lto = lfrom + 2 * PW_;
+  int *lcount = lto + 2 * PW_;
+  cT v[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = t + 256 * q, i = e / PW_, c = e % PW_;
+    v[q] = (i < w && c < w) ? A[(long)(c0 + i) * lda + c0 + c] : cT{0, 0};
+    lu[e] = v[q];
+  }
+  if (t < 64) {   // the interchanges as a list of moved rows: lane entry (original strip row, current row)
+    int id = -1, pos = -1, nlist = 0;
+    for (int i = 0; i < w; ++i) {
+      const int pr = piv[i] - c0;
+      if (pr == i) continue;
+      if (!__ballot(pos == i)) { if (t == nlist) { id = i; pos = i; } ++nlist; }
+      if (!__ballot(pos == pr)) { if (t == nlist) { id = pr; pos = pr; } ++nlist; }
+      const bool at_i = pos == i, at_p = pos == pr;
+      if (at_i) pos = pr;
+      else if (at_p) pos = i;
+    }
+    const bool mv = (t < nlist) && id != pos;
+    const unsigned long long mm = __ballot(mv);
+    const int slot = __popcll(mm & ((1ull << t) - 1ull));
+    if (mv) { lfrom[slot] = id; lto[slot] = pos; }
+    if (t == 0) *lcount = __popcll(mm);
+    if (perm) {   // every lane loads before any stores
+      int *pz = perm + (long)z * sP;
+      const int pv = mv ? pz[c0 + id] : 0;
+      if (mv) pz[c0 + pos] = pv;
+    }
+  }
+  ts_tri_inverses<T>(X, v, fin + PanelWS<T>::fin_inv_off());
+}
+constexpr size_t PP_FIN_LDS = (size_t)(3 * PW_ * PW_ + PW_) * 16;   // ts_tri_inverses' area (FP64 size)
+
 // Row interchanges of strips as gathers A[c0 + to] = A[c0 + from] over their
 // moved rows, per block of AP_COLS columns: every moved entry is loaded into
 // registers, then (after a barrier) stored.
@@ -1142,7 +1308,7 @@ static size_t getrf_ws_bytes(int n, int batch) {
 template <class T>
 static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long sA, int *ipiv, long sP,
                           int *perm, int *dinfo, hipStream_t s, char *ws = nullptr, bool lookahead = true,
-                          bool own_gemm = false) {
+                          bool own_gemm = false, bool pp = false) {
   typedef typename C2<T>::t cT;
   constexpr int PW = PanelW<T>::W;
   const dim3 b256(256);
@@ -1171,6 +1337,17 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // The strips of the outer block [k0, k0 + nb): tournament, interchanges on
   // the block's own columns, L21, then the block's remaining columns by TRSM
   // and the rank-w update. Touches only the block's columns (rows >= k0).
+  if (pp && pp_ws_bytes(n) > 2 * (long)Gmax * setb)
+    qpms_hip_die(__FILE__, __LINE__, __func__, "partial-pivoting workspace does not fit (order %d)", n);
+  if (pp) QH_LAUNCH(k_pp_init, dim3(batch), dim3(1), 0, s, ws, sW);
+  // partial pivoting: the strip in place, column by column, then its slot
+  auto pp_strip = [&](int c0, int w, hipStream_t st) {
+    const int G = (n - c0 + PP_ROWS - 1) / PP_ROWS;
+    for (int j = 0; j <= w; ++j)
+      QH_LAUNCH(k_pp_step<T>, dim3(G, 1, batch), b256, 0, st, n, c0, w, j, A, lda, sA, ws, sW, ipiv, sP, dinfo);
+    QH_LAUNCH(k_pp_finish<T>, dim3(1, 1, batch), b256, PP_FIN_LDS, st, n, c0, w, A, lda, sA, ws, sW,
+              (long)(fin0 - ws) + (long)((c0 % LU_NB) / PW) * finb, perm, sP);
+  };
   auto block_strips = [&](int k0, int nb, hipStream_t st, bool wcu) {
     constexpr int F = ts_fan<T>();
     const WholeCu<T> &W = whole_cu<T>();
@@ -1179,6 +1356,15 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
     for (int j0 = 0; j0 < nb; j0 += PW) {
       const int w = min(PW, nb - j0), c0 = k0 + j0;
       char *fin = fin0 + (long)(j0 / PW) * finb;
+      if (pp) {   // the strip factored in place (L21 included); interchanges on the block's other columns
+        pp_strip(c0, w, st);
+        if (c0 > k0)
+          QH_LAUNCH(k_ts_apply<T>, dim3((c0 - k0 + AP_COLS - 1) / AP_COLS, 1, batch), b256, l_apply, st, n, c0, w, A,
+                    lda, sA, fin, sW, k0, c0);
+        if (k0 + nb > c0 + w)
+          QH_LAUNCH(k_ts_apply<T>, dim3((k0 + nb - c0 - w + AP_COLS - 1) / AP_COLS, 1, batch), b256, l_apply, st, n,
+                    c0, w, A, lda, sA, fin, sW, c0 + w, k0 + nb);
+      } else {
       const int G = (n - c0 + TsRows<T>::R - 1) / TsRows<T>::R;
       QH_LAUNCH(k_ts_local<T>, dim3(G, 1, batch), bts, l_local, st, n, c0, w, A, lda, sA, ws, sW, setb,
                          (long)(fin - ws), G == 1 ? 1 : 0, ipiv, sP, perm, dinfo);
@@ -1191,9 +1377,10 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
       }
       QH_LAUNCH(k_ts_apply<T>, dim3((nb + AP_COLS - 1) / AP_COLS, 1, batch), b256, l_apply, st, n, c0, w, A,
                 lda, sA, fin, sW, k0, k0 + nb);
+      }
       const cT *linv = reinterpret_cast<const cT *>(fin + PanelWS<T>::fin_inv_off()), *uinv = linv + PW * PW;
       const int mr = n - c0 - w;
-      if (mr > 0)   // L21 = A21 U11^{-1}
+      if (mr > 0 && !pp)   // L21 = A21 U11^{-1}
         QH_LAUNCH((k_gemm_sub<T, true>), dim3(1, (mr + GT - 1) / GT, batch), b256, l_asg, st, mr, w, w,
                   A + (long)(c0 + w) * lda + c0, lda, sA, uinv, (long)PW, sWc, A + (long)(c0 + w) * lda + c0, lda,
                   sA);
@@ -1431,7 +1618,8 @@ extern "C" int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_d
   hipStream_t s = (hipStream_t)qh_stream(stream);
   int *dinfo = (int *)qh_dmalloc(sizeof(int));
   HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int), s));
-  getrf_batched<double>((int)n, 1, (v2d *)A_dev, (long)n, 0, ipiv_dev, 0, nullptr, dinfo, s);
+  getrf_batched<double>((int)n, 1, (v2d *)A_dev, (long)n, 0, ipiv_dev, 0, nullptr, dinfo, s, nullptr, true, false,
+                        /*pp=*/true);
   int info = 0;
   HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1442,8 +1630,8 @@ extern "C" int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_d
 // ---------------------------------------------------------------------------
 // Allocation-free variant of qpms_hip_zgetrf_dev for callers that cannot
 // allocate device memory (code under graph capture, serving loops that
-// forbid hipMalloc). Its scratch -- the info word and the tournament
-// workspace -- is the file-scope array below; the GEMMs are the
+// forbid hipMalloc). Its scratch -- the info word and the strip workspace
+// -- is the file-scope array below; the GEMMs are the
 // hand-written kernel (the library GEMM may allocate) and there is no
 // lookahead side stream. The host never names the array in a kernel
 // argument or copy: it uses the device address hipGetSymbolAddress returned
@@ -1490,7 +1678,7 @@ extern "C" int qpms_hip_zgetrf_dev_noalloc(size_t n, _Complex double *A_dev, int
   int *dinfo = (int *)base;
   qh_zero(dinfo, 256, s);   // a kernel: the info word must start at zero on every call
   getrf_batched<double>((int)n, 1, (v2d *)A_dev, (long)n, 0, ipiv_dev, 0, nullptr, dinfo, s, base + 256,
-                        /*lookahead=*/false, /*own_gemm=*/true);
+                        /*lookahead=*/false, /*own_gemm=*/true, /*pp=*/true);
   int info = 0;
   HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

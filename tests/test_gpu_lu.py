@@ -1,8 +1,13 @@
-"""GPU: the blocked LU (tournament-pivoted tall-strip panels, rank-256
-trailing updates) in FP64 and FP32, single and batched, across the
-outer-block and strip boundaries and the tournament's one- and two-level
-merges (strips of more than 16 x 512 rows), against P A = L U, numpy solves
-and zgetrf's info."""
+"""GPU: the blocked LU (tall-strip panels of 32 columns, rank-256 trailing
+updates) in FP64 and FP32, single and batched, across the outer-block and
+strip boundaries, against P A = L U, numpy solves and zgetrf's info.
+
+qpms_hip_zgetrf_dev (the factorisation the ABI returns, scatsystem.c:2214)
+uses partial pivoting: its interchanges equal host LAPACK's zgetrf (Intel MKL
+where present) and the oracle's unblocked zgetrf on generic matrices, and its
+factors agree to rounding. The batched LU (serving, Beyn nodes) uses
+tournament pivoting (strips of more than 8 x 256 rows go through one- and
+two-level merges); its factors satisfy P A = L U with their own pivots."""
 import ctypes
 
 import numpy as np
@@ -63,6 +68,23 @@ def test_zgetrf_pivots_match_lapack_when_unambiguous(gpu):
     assert info == infoo == 0
     assert np.array_equal(piv, pivo)
     assert np.max(np.abs(LU - LUo)) / np.max(np.abs(LUo)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [31, 257, 700, 2100])
+def test_zgetrf_pivots_equal_lapack(gpu, n):
+    """Generic random matrices (no designed margins): the interchanges are
+    zgetrf's, element for element, against host LAPACK and the oracle; the
+    factors agree to 1e-11 of their largest entry."""
+    import qpms_amd
+    A = rand(n, 900 + n)
+    LU, piv, info = qpms_amd.zgetrf(A)
+    LUh, pivh, infoh = oracle_ffi.host_lapack().zgetrf(A.copy())
+    LUo, pivo, infoo = oracle_ffi.zgetrf(A, nthreads=8)
+    assert info == infoh == infoo == 0
+    assert np.array_equal(pivh, pivo)
+    first = int(np.argmax(piv != pivh)) if not np.array_equal(piv, pivh) else -1
+    assert first < 0, (first, piv[first], pivh[first])
+    assert np.max(np.abs(LU - LUh)) / np.max(np.abs(LUh)) < 1e-11
 
 
 @pytest.mark.parametrize("n,zc", [(600, 300), (200, 0), (130, 129)])
