@@ -23,11 +23,13 @@ exactly the requests the plain path covered, so `vs_plain_fp64` compares the
 two paths on identical work. Every plain result is compared with the served
 one (extinction, log-determinant, Beyn singular values and eigenvalues).
 
-`roofline`: the FP32 LU of a mode-problem matrix of order 16380 (the loop's
-dominant work: tournament-pivoted panels plus trailing complex GEMMs on
-MFMA), timed live with HIP events on the library stream and priced at
-8/3 n^3 real flops against the FP32 matrix peak; `traffic` comes from the
-committed rocprofv3 counter summary of the same probe. `w_build`: the plain
+`roofline`: the FP32 LU of a mode-problem matrix of order 44880 -- the order
+of the largest request of the default timed segment, a 32-node contour whose
+node factorisations are most of the segment's GPU time: tournament-pivoted
+panels plus trailing complex GEMMs on MFMA -- timed live with HIP events on
+the library stream and priced at 8/3 n^3 real flops against the FP32 matrix
+peak; `traffic` comes from the committed rocprofv3 counter summary of the
+same probe. `roofline_lu16380` is the same at order 16380. `w_build`: the plain
 translation-matrix build (61440^2 complex128, HBM roofline). `cpu_baseline`:
 the reference's CPU path of the plane-wave solves of the timed requests (the
 oracle's pair loop for M = I - T W, host LAPACK zgetrf/zgetrs as
@@ -125,13 +127,7 @@ def w_probe(L):
     translation-matrix build of the N = 2048, lMax 3 workload."""
     import qpms_amd
     from qpms_amd.workloads import cluster_positions
-    n = 1 << 27
-    a, b = qpms_amd.DeviceBuffer(16 * n), qpms_amd.DeviceBuffer(16 * n)
-    L.qh_copy_c64.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-    L.qh_copy_c64(n, a.ptr, b.ptr, None)
-    L.qpms_hip_sync()
-    a.free()
-    b.free()
+    copy_calibration(L)
     pos = cluster_positions()
     calc = qpms_amd.TransCalculator(3)
     Ld = 30 * len(pos)
@@ -139,7 +135,7 @@ def w_probe(L):
     L.qpms_hip_build_translation_matrix_dev(calc.c, len(pos), dpos.ptr, qpms_amd.c_cplx.of(1.52 * 2.0 * EV / C0), 3,
                                             dW.ptr, None)
     L.qpms_hip_sync()
-    print(json.dumps({"w_probe": "done", "copy_bytes_each_way": 16 * n, "w_bytes": 16 * Ld * Ld}), flush=True)
+    print(json.dumps({"w_probe": "done", "copy_bytes_each_way": 16 << 27, "w_bytes": 16 * Ld * Ld}), flush=True)
 
 
 def measure_peaks(L, ev, stream):
@@ -204,11 +200,43 @@ def measure_peaks(L, ev, stream):
     return out
 
 
-def measure_lu(L, ev, stream, n_part=546, reps=3):
-    """FP32 LU of a mode-problem matrix of order 30 * n_part (16380 by
-    default; the serving loop's dominant work), HIP events on the library
-    stream."""
+def autotune(L):
+    """Runs the library's GEMM autotuner (trailing-update GEMM per size class:
+    rocBLAS or one of the hand-written 3M / 4M MFMA kernels, the fastest on
+    this device) and returns its table."""
+    L.qpms_hip_autotune.argtypes = [ctypes.c_void_p]
+    t0 = time.perf_counter()
+    L.qpms_hip_autotune(None)
+    buf = ctypes.create_string_buffer(4096)
+    L.qpms_hip_autotune_table(buf, ctypes.c_size_t(4096))
+    table = {}
+    for ln in buf.value.decode().splitlines():
+        b, d, v = ln.split()
+        table[f"{'batched' if b == '1' else 'single'}_{d}"] = v
+    log(f"GEMM autotune ({time.perf_counter() - t0:.1f} s): " + ", ".join(f"{k} {v}" for k, v in table.items()))
+    return table
+
+
+def copy_calibration(L):
+    """One plain copy kernel over 2 GiB (k_copy2d reads and writes 2 GiB):
+    the FETCH_SIZE / WRITE_SIZE calibration of a counter pass."""
     import qpms_amd
+    n = 1 << 27
+    a, b = qpms_amd.DeviceBuffer(16 * n), qpms_amd.DeviceBuffer(16 * n)
+    L.qh_copy_c64.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.qh_copy_c64(n, a.ptr, b.ptr, None)
+    L.qpms_hip_sync()
+    a.free()
+    b.free()
+
+
+def measure_lu(L, ev, stream, n_part=1496, reps=3, calib=False):
+    """FP32 LU of a mode-problem matrix of order 30 * n_part (44880 by
+    default: the order of the timed segment's largest request), HIP events on
+    the library stream. calib: a copy kernel first (counter passes)."""
+    import qpms_amd
+    if calib:
+        copy_calibration(L)
     rng = np.random.default_rng(2)
     n = 30 * n_part
     side = (n_part ** (1 / 3)) * 300e-9
@@ -235,6 +263,7 @@ def measure_lu(L, ev, stream, n_part=546, reps=3):
         L.qh_getrf_batched_f32(n, 1, dF.ptr, 0, dp.ptr, dpm.ptr, dinfo.ptr, None)
         ev.record(b, stream)
         ms = ev.elapsed_ms(a, b)
+        log(f"LU of order {n}: factorisation {r + 1} of {reps + 1} ({'timed' if r else 'warm-up'}) {ms:.1f} ms")
         if r:
             times.append(ms)
     for x in (dM, dF, dp, dpm, dinfo, dpos, dty, dtd):
@@ -246,7 +275,7 @@ def measure_lu(L, ev, stream, n_part=546, reps=3):
             "frac": ach / FP32_MFMA_PEAK,
             # HBM bytes per factorisation from the two rocprofv3 --pmc passes of
             # this probe (FETCH_SIZE, WRITE_SIZE, copy-kernel calibration)
-            "traffic": profile_traffic("lu_fp32_order16380") if n == 16380 else None,
+            "traffic": profile_traffic(f"lu_fp32_order{n}"),
             "kernel": f"FP32 LU, order {n} (tournament panels + trailing GEMM)",
             "avg_launch_ms": avg, "algorithmic_flops_per_launch": flops}
 
@@ -652,8 +681,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-wbuild", action="store_true")
     ap.add_argument("--lu-probe", action="store_true", help="only the FP32 LU measurement (for rocprofv3)")
-    ap.add_argument("--lu-particles", type=int, default=546,
-                    help="particles (lMax 3) of the LU probe's matrix: order 30 x this (546: 16380)")
+    ap.add_argument("--lu-particles", type=int, default=1496,
+                    help="particles (lMax 3) of the LU probe's matrix: order 30 x this (1496: 44880, 546: 16380); "
+                         "the probe runs a 2 GiB copy kernel first (counter calibration)")
+    ap.add_argument("--lu-reps", type=int, default=2, help="timed factorisations of the LU probe (after one warm-up)")
     ap.add_argument("--w-probe", action="store_true",
                     help="one 2 GiB copy kernel (counter calibration) and one N = 2048 W build (for rocprofv3 --pmc)")
     ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
@@ -676,7 +707,9 @@ def main():
         w_probe(L)
         return
     if args.lu_probe:
-        print(json.dumps(measure_lu(L, ev, stream, n_part=args.lu_particles, reps=2)), flush=True)
+        autotune(L)
+        print(json.dumps(measure_lu(L, ev, stream, n_part=args.lu_particles, reps=args.lu_reps, calib=True)),
+              flush=True)
         return
     if args.burst:
         print(json.dumps(burst(args.burst)), flush=True)
@@ -702,8 +735,11 @@ def main():
     wb = None if args.no_wbuild else measure_w_build(L, ev, stream)
     if wb:
         log(f"W build: {wb['achieved']:.0f} GB/s ({100 * wb['frac']:.1f} % of HBM peak)")
-    roof = measure_lu(L, ev, stream)
-    log(f"FP32 LU n=16380: {roof['avg_launch_ms']:.1f} ms, {roof['achieved']:.1f} TF/s")
+    tune = autotune(L)
+    roof = measure_lu(L, ev, stream, n_part=1496, reps=2)
+    log(f"FP32 LU n=44880: {roof['avg_launch_ms']:.1f} ms, {roof['achieved']:.1f} TF/s")
+    roof16 = measure_lu(L, ev, stream, n_part=546, reps=3)
+    log(f"FP32 LU n=16380: {roof16['avg_launch_ms']:.1f} ms, {roof16['achieved']:.1f} TF/s")
 
     S, K, W = args.slice, args.steps, args.warmup
     reqs = sv.serving_stream(n=10000, rate=args.rate)
@@ -797,6 +833,8 @@ def main():
         "plain_fp64": plain,
         "vs_plain_fp64": (plain["plain_busy_s"] / plain["served_wall_s"]) if plain else None,
         "roofline": roof,
+        "roofline_lu16380": roof16,
+        "gemm_autotune": tune,
         "w_build": wb,
         "measured_peaks": peaks,
         "cpu_baseline": cpu,

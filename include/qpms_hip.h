@@ -699,9 +699,14 @@ int qpms_hip_zgetrs_dev(size_t n, size_t nrhs, const _Complex double *LU_dev, co
  * times the library GEMM and the hand-written kernels (3M MFMA kernel in
  * three tile shapes, four-product kernel) per size class on scratch operands
  * (allocated and freed here) and records the fastest for every later LU in
- * the process. Returns the number of classes tuned; `report` (optional,
- * 2 x 6 x 5 floats) receives the milliseconds [batched][class][variant].
- * The serving loop runs it at creation when QPMS_HIP_AUTOTUNE is set. */
+ * the process; the single-matrix classes of order 2048 to 16384 are then
+ * decided by timing whole FP32 factorisations of that order (on a stream of
+ * the tuner's own), with the class's updates switched between the library
+ * and the two fastest 3M tiles. A hand-written variant within 3 % of the
+ * library is preferred. Returns the number of GEMM-timed classes; `report`
+ * (optional, 2 x 6 x 5 floats) receives the milliseconds
+ * [batched][class][variant]. The serving loop runs it at creation when
+ * QPMS_HIP_AUTOTUNE is set. */
 int qpms_hip_autotune(float *report);
 /* The tuned table as text ("batched class-dim variant" per line). */
 int qpms_hip_autotune_table(char *buf, size_t len);

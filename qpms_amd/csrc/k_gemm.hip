@@ -244,11 +244,14 @@ namespace {
 constexpr int NCLASS = 6;
 constexpr int CLASS_DIM[NCLASS] = {768, 2048, 4096, 8192, 16384, 32768};   // representative (upper) dims
 constexpr int NVAR = 5;
+constexpr float OWN_MARGIN = 1.03f;   // preference for a hand-written variant over the library
 const char *const VAR_NAME[NVAR] = {"rocblas", "3m_128x128", "3m_128x64", "3m_64x64", "own_4m"};
 std::mutex g_tune_mu;
 // [batched][class] -> variant; -1: not tuned
 int g_tuned[2][NCLASS] = {{-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}};
 float g_tuned_ms[2][NCLASS][NVAR];
+// while whole factorisations are timed: single-matrix updates of this class use this variant
+int g_override_class = -1, g_override_var = -1;
 
 int size_class(int M, int N) {
   const int d = M > N ? M : N;
@@ -263,6 +266,8 @@ int forced_variant() {
     if (!e) return -1;
     if (!strcmp(e, "rocblas")) return 0;
     if (!strcmp(e, "3m")) return 1;
+    if (!strcmp(e, "3m128x64")) return 2;
+    if (!strcmp(e, "3m64")) return 3;
     if (!strcmp(e, "own")) return 4;
     return -1;
   }();
@@ -277,6 +282,7 @@ extern "C" int qh_lu_gemm_variant(int M, int N, int batched) {
   if (f >= 0) return f;
   std::lock_guard<std::mutex> g(g_tune_mu);
   const int c = size_class(M, N), t = g_tuned[batched ? 1 : 0][c];
+  if (!batched && c == g_override_class) return g_override_var;
   if (t >= 0) return t;
   return (M >= 512 && N >= 512) ? 0 : 4;   // untuned: the library GEMM for large updates
 }
@@ -318,8 +324,9 @@ extern "C" int qh_lu_gemm_run(int v, int M, int N, int K, int batch, const float
 }
 
 // Times every candidate for every size class (batched: 4 matrices of the
-// class's size / 2) on scratch operands and records the fastest. Returns the
-// number of classes tuned. `report` (optional, >= 2 * 6 * 5 floats) receives
+// class's size / 2) on scratch operands and records the fastest; the
+// single-matrix classes of order 2048 and above are then re-decided by whole
+// factorisations (qh_autotune_lu). Returns the number of GEMM-timed classes. `report` (optional, >= 2 * 6 * 5 floats) receives
 // the measured milliseconds [batched][class][variant].
 extern "C" int qpms_hip_autotune(float *report) {
   hipStream_t s;
@@ -366,6 +373,14 @@ extern "C" int qpms_hip_autotune(float *report) {
         g_tuned_ms[bt][c][v] = ms;
         if (ms < bestms) { bestms = ms; best = v; }
       }
+      // a hand-written variant within OWN_MARGIN of the library is preferred
+      // (it also runs inside captured graphs and needs no library workspace)
+      if (best == 0) {
+        int own = -1;
+        for (int v = 1; v < NVAR; ++v)
+          if (own < 0 || g_tuned_ms[bt][c][v] < g_tuned_ms[bt][c][own]) own = v;
+        if (own > 0 && g_tuned_ms[bt][c][own] <= OWN_MARGIN * bestms) best = own;
+      }
       {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tuned[bt][c] = best;
@@ -379,6 +394,47 @@ extern "C" int qpms_hip_autotune(float *report) {
   HIPCHK(hipEventDestroy(e1));
   rocblas_destroy_handle(h);
   HIPCHK(hipStreamDestroy(s));
+  qh_autotune_lu(nullptr);   // the large single-matrix classes, by whole factorisations
+  return tuned;
+}
+
+// Whole-factorisation tuning of the single-matrix classes from LU_TUNE_MIN on:
+// the trailing update of a blocked LU runs beside the panel chain (on a
+// CU-masked stream, next to whole-CU strip kernels), where the ranking of
+// the candidates can differ from that of an isolated GEMM. For each such
+// class the FP32 LU of a matrix of the class's order is timed once per
+// candidate (rocBLAS, 3M 128 x 64, 3M 64 x 64), with only that class's
+// updates switched, and the fastest is recorded. lu(n, ms) factors a fresh
+// matrix of order n and returns the milliseconds. As for the GEMM timings, a
+// hand-written candidate within OWN_MARGIN of the library is preferred.
+constexpr int LU_TUNE_MIN = 2048;
+extern "C" int qh_autotune_lu_classes(float (*lu)(int n, void *ctx), void *ctx, float *report) {
+  const int cand[3] = {0, 2, 3};
+  int tuned = 0;
+  for (int c = 0; c < NCLASS; ++c) {
+    const int d = CLASS_DIM[c];
+    if (d < LU_TUNE_MIN) continue;
+    float ms[3];
+    for (int k = 0; k < 3; ++k) {
+      {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_override_class = c;
+        g_override_var = cand[k];
+      }
+      ms[k] = lu(d, ctx);
+      if (report) report[c * 3 + k] = ms[k];
+    }
+    int best = -1;
+    for (int k = 0; k < 3; ++k)
+      if (ms[k] > 0 && (best < 0 || ms[k] < ms[best])) best = k;
+    if (best == 0) {   // the faster hand-written candidate, if within OWN_MARGIN of the library
+      const int own = (ms[2] > 0 && (ms[1] <= 0 || ms[2] < ms[1])) ? 2 : 1;
+      if (ms[own] > 0 && ms[own] <= OWN_MARGIN * ms[0]) best = own;
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_override_class = g_override_var = -1;
+    if (best >= 0) { g_tuned[0][c] = cand[best]; ++tuned; }
+  }
   return tuned;
 }
 

@@ -1051,11 +1051,11 @@ static bool use_rocblas() {
 // stream (the serving loop's lanes) must call it first, so that a later
 // stream created at the same address never inherits a dead stream's state.
 struct LaCtx {
-  hipStream_t side[2];   // the trailing update beyond the next block, per order class (below / from LA_BIG)
-  bool made[2], masked[2];
+  hipStream_t side;   // the trailing update beyond the next block (all CUs but the reserved ones)
+  bool masked;        // the side stream leaves CUs free
   hipEvent_t go, done;
 };
-constexpr int LA_BIG = 24576;   // orders from here on keep fewer CUs free (their GEMM is the longer part)
+
 constexpr int STREAM_TAB = 64;
 struct StreamState {
   hipStream_t key;
@@ -1093,46 +1093,42 @@ static rocblas_handle blas_for(hipStream_t s) {
 
 // CUs kept free of the side stream's trailing update, so that the next
 // block's strips (latency-bound launches of tens of workgroups, launched as
-// whole-CU workgroups) find CUs while it runs: QPMS_HIP_LU_RESERVE for orders
-// below LA_BIG (default 64), QPMS_HIP_LU_RESERVE_BIG from it on (default 32);
-// 0: no CU mask (the strips then wait for CUs to drain).
-static int lu_reserve_cus(bool big) {
-  static int v[2] = {-1, -1};
-  if (v[big] < 0) {
-    const char *e = getenv(big ? "QPMS_HIP_LU_RESERVE_BIG" : "QPMS_HIP_LU_RESERVE");
-    v[big] = e ? std::max(0, atoi(e)) : (big ? 32 : 64);
+// whole-CU workgroups) find CUs while it runs: QPMS_HIP_LU_RESERVE (default
+// 32; 0: no CU mask, the strips then wait for CUs to drain). One masked side
+// stream per caller stream: a second, differently masked one measured slower.
+static int lu_reserve_cus() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("QPMS_HIP_LU_RESERVE");
+    v = e ? std::max(0, atoi(e)) : 32;
   }
-  return v[big];
+  return v;
 }
 
-// Lookahead context of a caller stream: per order class a side stream,
-// masked to all CUs but the reserved ones, that runs the bulk of each
-// trailing update while the caller's stream factors the next block's strips,
-// and the two events ordering them (default priority; created on first use).
-static LaCtx *la_for(hipStream_t s, bool big) {
+// Lookahead context of a caller stream: the side stream, masked to all CUs
+// but the reserved ones, that runs the bulk of each trailing update while
+// the caller's stream factors the next block's strips, and the two events
+// ordering them (default priority; created on first use).
+static LaCtx *la_for(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_tab_mu);
   StreamState *e = state_for(s);
   if (!e->la) {
-    e->lc.made[0] = e->lc.made[1] = false;
-    HIPCHK(hipEventCreateWithFlags(&e->lc.go, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&e->lc.done, hipEventDisableTiming));
-    e->la = true;
-  }
-  if (!e->lc.made[big]) {
-    const int reserve = lu_reserve_cus(big);
+    const int reserve = lu_reserve_cus();
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (reserve > 0 && ncu > 2 * reserve && ncu <= 32 * 32) {
       uint32_t mask[32] = {0};
       for (int i = 0; i < ncu - reserve; ++i) mask[i / 32] |= 1u << (i % 32);
-      HIPCHK(hipExtStreamCreateWithCUMask(&e->lc.side[big], (uint32_t)((ncu + 31) / 32), mask));
-      e->lc.masked[big] = true;
+      HIPCHK(hipExtStreamCreateWithCUMask(&e->lc.side, (uint32_t)((ncu + 31) / 32), mask));
+      e->lc.masked = true;
     } else {
-      HIPCHK(hipStreamCreateWithFlags(&e->lc.side[big], hipStreamNonBlocking));
-      e->lc.masked[big] = false;
+      HIPCHK(hipStreamCreateWithFlags(&e->lc.side, hipStreamNonBlocking));
+      e->lc.masked = false;
     }
-    e->lc.made[big] = true;
+    HIPCHK(hipEventCreateWithFlags(&e->lc.go, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->lc.done, hipEventDisableTiming));
+    e->la = true;
   }
   return &e->lc;
 }
@@ -1145,27 +1141,23 @@ extern "C" void qh_stream_release(void *stream) {
   for (int i = 0; i < g_ntab; ++i) {
     if (g_tab[i].key != s) continue;
     StreamState &e = g_tab[i];
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t side = nullptr;
     if (e.la) {
-      for (int c = 0; c < 2; ++c)
-        if (e.lc.made[c]) {
-          side[c] = e.lc.side[c];
-          HIPCHK(hipStreamSynchronize(side[c]));
-        }
+      side = e.lc.side;
+      HIPCHK(hipStreamSynchronize(side));
       HIPCHK(hipEventDestroy(e.lc.go));
       HIPCHK(hipEventDestroy(e.lc.done));
     }
     if (e.blas) rocblas_destroy_handle(e.blas);
     g_tab[i] = g_tab[--g_ntab];
-    for (int c = 0; c < 2; ++c) {   // each side stream's own state (its rocBLAS handle), then the stream
-      if (!side[c]) continue;
+    if (side) {   // the side stream's own state (its rocBLAS handle), then the stream
       for (int k = 0; k < g_ntab; ++k) {
-        if (g_tab[k].key != side[c]) continue;
+        if (g_tab[k].key != side) continue;
         if (g_tab[k].blas) rocblas_destroy_handle(g_tab[k].blas);
         g_tab[k] = g_tab[--g_ntab];
         break;
       }
-      HIPCHK(hipStreamDestroy(side[c]));
+      HIPCHK(hipStreamDestroy(side));
     }
     return;
   }
@@ -1192,7 +1184,8 @@ static void trailing_gemm(int M, int N, int K, int batch, const typename C2<T>::
   // record allocations or memsets into the graph)
   const bool capturing = stream_capturing(s);
   if (capturing) own_only = true;
-  if (sizeof(T) == 4 && (!own_only || capturing)) {   // FP32: the variant the autotuner picked for this size class
+  if (sizeof(T) == 4 && (!own_only || capturing)) {
+    // FP32: the variant the autotuner picked for this size class
     int v = qh_lu_gemm_variant(M, N, batch > 1);
     if (v == 0 && capturing) v = qh_lu_gemm_best_own(M, N, batch > 1);
     qh_lu_gemm_run(v, M, N, K, batch, (const float *)A, lda, sA, (const float *)B, ldb, sB, (float *)C, ldc, sC,
@@ -1402,8 +1395,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // interchanges are applied to the columns outside it afterwards (the strips
   // touch only their own block), after the side stream's update.
   const bool la = lookahead && use_lookahead() && n > 2 * LU_NB && !stream_capturing(s);
-  const bool big = n >= LA_BIG;
-  LaCtx *cx = la ? la_for(s, big) : nullptr;
+  LaCtx *cx = la ? la_for(s) : nullptr;
   if (perm) QH_LAUNCH(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
   block_strips(0, min(LU_NB, n), s, false);
   for (int k0 = 0; k0 < n; k0 += LU_NB) {
@@ -1431,7 +1423,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
                        sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s, own_gemm);
       const bool split = la && nr > nb1;
       if (nr > nb1) {
-        hipStream_t sg = split ? cx->side[big] : s;
+        hipStream_t sg = split ? cx->side : s;
         if (split) {
           HIPCHK(hipEventRecord(cx->go, s));
           HIPCHK(hipStreamWaitEvent(sg, cx->go, 0));
@@ -1441,7 +1433,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
                          sA, sg, own_gemm);
         if (split) HIPCHK(hipEventRecord(cx->done, sg));
       }
-      block_strips(k0 + nb, nb1, s, split && cx->masked[big]);
+      block_strips(k0 + nb, nb1, s, split && cx->masked);
       if (split) HIPCHK(hipStreamWaitEvent(s, cx->done, 0));
     }
   }
@@ -1625,6 +1617,90 @@ extern "C" int qpms_hip_zgetrf_dev(size_t n, _Complex double *A_dev, int *ipiv_d
   HIPCHK(hipStreamSynchronize(s));
   qh_dfree(dinfo);
   return info;
+}
+
+// ---------------------------------------------------------------------------
+// Whole-factorisation timing for the GEMM autotuner: FP32 LUs of a
+// well-conditioned pseudo-random matrix (entries in [-1/2, 1/2]^2 from a hash
+// of the indices, 3 sqrt(n) added on the diagonal), with lookahead, as the
+// serving loop and the LU entry points run them, on a stream of their own.
+// ---------------------------------------------------------------------------
+__global__ void k_tune_fill(long n, v2f *A) {
+  const long tot = n * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    unsigned long long h = (unsigned long long)e * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    const float re = (float)(h & 0xFFFFFF) * (1.f / 16777216.f) - 0.5f;
+    const float im = (float)((h >> 24) & 0xFFFFFF) * (1.f / 16777216.f) - 0.5f;
+    A[e] = v2f{re + ((e / n == e % n) ? 3.f * sqrtf((float)n) : 0.f), im};
+  }
+}
+
+namespace {
+struct TuneCtx {
+  v2f *A;
+  char *ws;
+  int *ipiv, *info;
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  int nmax;
+};
+float tune_lu(int n, void *p) {
+  TuneCtx &c = *static_cast<TuneCtx *>(p);
+  if (n > c.nmax) return -1.f;
+  float best = -1.f;
+  for (int rep = 0; rep < 2; ++rep) {   // the first run warms the shapes' kernels up
+    QH_LAUNCH(k_tune_fill, dim3(qh_stride_blocks((long)n * n)), dim3(256), 0, c.s, (long)n, c.A);
+    HIPCHK(hipEventRecord(c.e0, c.s));
+    getrf_batched<float>(n, 1, c.A, n, 0, c.ipiv, n, nullptr, c.info, c.s, c.ws, true);
+    HIPCHK(hipEventRecord(c.e1, c.s));
+    HIPCHK(hipEventSynchronize(c.e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.e0, c.e1));
+    if (rep) best = ms;
+  }
+  return best;
+}
+}  // namespace
+
+// Tunes the single-matrix classes by whole factorisations (largest order
+// that fits the free memory, at most 16384; the class above keeps its GEMM
+// timing). Returns the classes tuned.
+extern "C" int qh_autotune_lu(float *report) {
+  TuneCtx c{};
+  HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));   // released with its LU state at the end
+  for (int n = 16384; n >= 2048 && !c.A; n /= 2) {
+    void *a = nullptr, *w = nullptr;
+    if (hipMalloc(&a, (size_t)n * n * 8) != hipSuccess ||
+        hipMalloc(&w, getrf_ws_bytes<float>(n, 1) + 8 * (size_t)n + 256) != hipSuccess) {
+      (void)hipGetLastError();
+      if (a) HIPCHK(hipFree(a));
+      if (w) HIPCHK(hipFree(w));
+      continue;
+    }
+    c.A = (v2f *)a;
+    c.ws = (char *)w;
+    c.nmax = n;
+  }
+  if (!c.A) {
+    HIPCHK(hipStreamDestroy(c.s));
+    return 0;
+  }
+  c.ipiv = (int *)(c.ws + getrf_ws_bytes<float>(c.nmax, 1));
+  c.info = c.ipiv + c.nmax;
+  HIPCHK(hipEventCreate(&c.e0));
+  HIPCHK(hipEventCreate(&c.e1));
+  const int tuned = qh_autotune_lu_classes(tune_lu, &c, report);
+  HIPCHK(hipStreamSynchronize(c.s));
+  HIPCHK(hipEventDestroy(c.e0));
+  HIPCHK(hipEventDestroy(c.e1));
+  HIPCHK(hipFree(c.A));
+  HIPCHK(hipFree(c.ws));
+  qh_stream_release(c.s);
+  HIPCHK(hipStreamDestroy(c.s));
+  return tuned;
 }
 
 // ---------------------------------------------------------------------------

@@ -88,6 +88,10 @@ int qh_lu_gemm_variant(int M, int N, int batched);
 int qh_lu_gemm_best_own(int M, int N, int batched);
 int qh_lu_gemm_run(int v, int M, int N, int K, int batch, const float *A, long lda, long sA,
 		const float *B, long ldb, long sB, float *C, long ldc, long sC, void *blas, void *stream);
+/* Whole-factorisation tuning of the trailing-update variant per size class
+ * (k_gemm.hip drives, k_linalg.hip times the LUs). */
+int qh_autotune_lu_classes(float (*lu)(int n, void *ctx), void *ctx, float *report);
+int qh_autotune_lu(float *report);
 int qh_d2f(long n, const double *x, float *y, void *stream);
 int qh_f2d_add(long n, const float *d, double *x, void *stream);
 int qh_copy_c64(long n, const double *x, double *y, void *stream);
