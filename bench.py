@@ -685,6 +685,9 @@ def main():
                     help="particles (lMax 3) of the LU probe's matrix: order 30 x this (1496: 44880, 546: 16380); "
                          "the probe runs a 2 GiB copy kernel first (counter calibration)")
     ap.add_argument("--lu-reps", type=int, default=2, help="timed factorisations of the LU probe (after one warm-up)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="LU probe without the GEMM autotuner (counter passes: the tuner's own factorisations "
+                         "crash rocprofv3 --pmc; QPMS_HIP_GEMM then names the variant)")
     ap.add_argument("--w-probe", action="store_true",
                     help="one 2 GiB copy kernel (counter calibration) and one N = 2048 W build (for rocprofv3 --pmc)")
     ap.add_argument("--per-kind", type=int, default=0, help="serve the first N requests one at a time vs plain")
@@ -707,7 +710,8 @@ def main():
         w_probe(L)
         return
     if args.lu_probe:
-        autotune(L)
+        if not args.no_autotune:
+            autotune(L)
         print(json.dumps(measure_lu(L, ev, stream, n_part=args.lu_particles, reps=args.lu_reps, calib=True)),
               flush=True)
         return

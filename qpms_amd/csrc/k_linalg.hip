@@ -1051,9 +1051,7 @@ static bool use_rocblas() {
 // stream (the serving loop's lanes) must call it first, so that a later
 // stream created at the same address never inherits a dead stream's state.
 struct LaCtx {
-  hipStream_t side;   // the trailing update beyond the next block (all CUs but the reserved ones)
-  bool masked;        // the side stream leaves CUs free
-  hipEvent_t go, done;
+  hipEvent_t go, done;   // ordering of the caller's stream against the device's side stream
 };
 
 constexpr int STREAM_TAB = 64;
@@ -1094,8 +1092,7 @@ static rocblas_handle blas_for(hipStream_t s) {
 // CUs kept free of the side stream's trailing update, so that the next
 // block's strips (latency-bound launches of tens of workgroups, launched as
 // whole-CU workgroups) find CUs while it runs: QPMS_HIP_LU_RESERVE (default
-// 32; 0: no CU mask, the strips then wait for CUs to drain). One masked side
-// stream per caller stream: a second, differently masked one measured slower.
+// 32; 0: no CU mask, the strips then wait for CUs to drain).
 static int lu_reserve_cus() {
   static int v = -1;
   if (v < 0) {
@@ -1105,60 +1102,65 @@ static int lu_reserve_cus() {
   return v;
 }
 
-// Lookahead context of a caller stream: the side stream, masked to all CUs
-// but the reserved ones, that runs the bulk of each trailing update while
-// the caller's stream factors the next block's strips, and the two events
-// ordering them (default priority; created on first use).
-static LaCtx *la_for(hipStream_t s) {
-  std::lock_guard<std::mutex> g(g_tab_mu);
-  StreamState *e = state_for(s);
-  if (!e->la) {
+// The side stream of the lookahead: one per device for the whole process,
+// masked to all CUs but the reserved ones, shared by every caller stream (the
+// bulk updates of concurrent factorisations queue on it, each ordered against
+// its own caller by that caller's events). Every further hardware queue
+// costs: with several masked side streams alive, factorisations measured up
+// to 1.6 x slower, depending on the order the queues were created in.
+constexpr int LA_MAX_DEV = 64;
+static hipStream_t g_la_side[LA_MAX_DEV];
+static bool g_la_masked[LA_MAX_DEV];
+static hipStream_t la_side(bool *masked) {   // caller holds g_tab_mu
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= LA_MAX_DEV) qpms_hip_die(__FILE__, __LINE__, __func__, "device %d outside the table", dev);
+  if (!g_la_side[dev]) {
     const int reserve = lu_reserve_cus();
-    int dev = 0, ncu = 0;
-    HIPCHK(hipGetDevice(&dev));
+    int ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (reserve > 0 && ncu > 2 * reserve && ncu <= 32 * 32) {
       uint32_t mask[32] = {0};
       for (int i = 0; i < ncu - reserve; ++i) mask[i / 32] |= 1u << (i % 32);
-      HIPCHK(hipExtStreamCreateWithCUMask(&e->lc.side, (uint32_t)((ncu + 31) / 32), mask));
-      e->lc.masked = true;
+      HIPCHK(hipExtStreamCreateWithCUMask(&g_la_side[dev], (uint32_t)((ncu + 31) / 32), mask));
+      g_la_masked[dev] = true;
     } else {
-      HIPCHK(hipStreamCreateWithFlags(&e->lc.side, hipStreamNonBlocking));
-      e->lc.masked = false;
+      HIPCHK(hipStreamCreateWithFlags(&g_la_side[dev], hipStreamNonBlocking));
+      g_la_masked[dev] = false;
     }
+  }
+  *masked = g_la_masked[dev];
+  return g_la_side[dev];
+}
+
+// Lookahead context of a caller stream: its two events (created on first use).
+static LaCtx *la_for(hipStream_t s, hipStream_t *side, bool *masked) {
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  StreamState *e = state_for(s);
+  if (!e->la) {
     HIPCHK(hipEventCreateWithFlags(&e->lc.go, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&e->lc.done, hipEventDisableTiming));
     e->la = true;
   }
+  *side = la_side(masked);
   return &e->lc;
 }
 
-// Destroys the per-stream state of `stream` (rocBLAS handle, lookahead side
-// stream and events), after waiting for the work queued on them.
+// Destroys the per-stream state of `stream` (rocBLAS handle, lookahead
+// events), after waiting for its work on the shared side stream.
 extern "C" void qh_stream_release(void *stream) {
   hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> g(g_tab_mu);
   for (int i = 0; i < g_ntab; ++i) {
     if (g_tab[i].key != s) continue;
     StreamState &e = g_tab[i];
-    hipStream_t side = nullptr;
-    if (e.la) {
-      side = e.lc.side;
-      HIPCHK(hipStreamSynchronize(side));
+    if (e.la) {   // the caller's own work on the shared side stream ends before its events go
+      HIPCHK(hipEventSynchronize(e.lc.done));
       HIPCHK(hipEventDestroy(e.lc.go));
       HIPCHK(hipEventDestroy(e.lc.done));
     }
     if (e.blas) rocblas_destroy_handle(e.blas);
     g_tab[i] = g_tab[--g_ntab];
-    if (side) {   // the side stream's own state (its rocBLAS handle), then the stream
-      for (int k = 0; k < g_ntab; ++k) {
-        if (g_tab[k].key != side) continue;
-        if (g_tab[k].blas) rocblas_destroy_handle(g_tab[k].blas);
-        g_tab[k] = g_tab[--g_ntab];
-        break;
-      }
-      HIPCHK(hipStreamDestroy(side));
-    }
     return;
   }
 }
@@ -1393,9 +1395,14 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
   // k+1's strips run here; the side stream's update is enqueued first, so it
   // starts at once, and the strips find the reserved CUs free. Block k+1's
   // interchanges are applied to the columns outside it afterwards (the strips
-  // touch only their own block), after the side stream's update.
-  const bool la = lookahead && use_lookahead() && n > 2 * LU_NB && !stream_capturing(s);
-  LaCtx *cx = la ? la_for(s) : nullptr;
+  // touch only their own block), after the side stream's update. Single
+  // matrices only: a batch's strips already fill the device (batch x tens of
+  // workgroups), and as whole-CU workgroups on the reserved CUs they would
+  // queue (a batch of 8 at order 9216: 274 us per tournament launch).
+  const bool la = lookahead && use_lookahead() && batch == 1 && n > 2 * LU_NB && !stream_capturing(s);
+  hipStream_t side = nullptr;
+  bool masked = false;
+  LaCtx *cx = la ? la_for(s, &side, &masked) : nullptr;
   if (perm) QH_LAUNCH(k_iota, dim3((n + 255) / 256, 1, batch), b256, 0, s, n, perm, sP);
   block_strips(0, min(LU_NB, n), s, false);
   for (int k0 = 0; k0 < n; k0 += LU_NB) {
@@ -1423,7 +1430,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
                        sA, A + (long)(k0 + nb) * lda + k0 + nb, lda, sA, s, own_gemm);
       const bool split = la && nr > nb1;
       if (nr > nb1) {
-        hipStream_t sg = split ? cx->side : s;
+        hipStream_t sg = split ? side : s;
         if (split) {
           HIPCHK(hipEventRecord(cx->go, s));
           HIPCHK(hipStreamWaitEvent(sg, cx->go, 0));
@@ -1433,7 +1440,7 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
                          sA, sg, own_gemm);
         if (split) HIPCHK(hipEventRecord(cx->done, sg));
       }
-      block_strips(k0 + nb, nb1, s, split && cx->masked);
+      block_strips(k0 + nb, nb1, s, split && masked);
       if (split) HIPCHK(hipStreamWaitEvent(s, cx->done, 0));
     }
   }

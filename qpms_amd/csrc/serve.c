@@ -63,12 +63,14 @@ int qh_getrf_batched_wsla_f32(int n, int batch, float *A, long sA, int *ipiv, in
 		int lookahead, void *stream);
 int qh_getrf_batched_wsla_f64(int n, int batch, double *A, long sA, int *ipiv, int *perm, int *info, void *ws,
 		int lookahead, void *stream);
-/* Orders from which a served LU runs its strips on a lookahead side stream
- * (the strip chain then overlaps the trailing GEMM), and up to which Beyn
- * chunks are replayed from a captured graph (launch-bound orders; larger
- * ones are compute-bound and keep the lookahead, which a capture excludes). */
+/* Orders from which a served single-matrix LU uses the lookahead (the bulk
+ * of each trailing update on the device's side stream beside the next
+ * block's strips), and up to which Beyn chunks are replayed from a captured
+ * graph (launch-bound orders; a capture excludes the library GEMM and the
+ * lookahead, which cost more than the launches from order ~4096 on: a
+ * 96-node contour of order 9216 took 4.7 s captured against 4.0 s eager). */
 #define SRV_LOOKAHEAD_MIN 4096
-#define SRV_GRAPH_MAX 12288
+#define SRV_GRAPH_MAX 4096
 int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
 		double *A0, double *A1, double *A0c, double *A1c, void *stream);
 int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
