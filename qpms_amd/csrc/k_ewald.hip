@@ -130,16 +130,19 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
 //   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
 // s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
 // ---------------------------------------------------------------------------
-// The terms of one K point without the phase e^{i K.s}: f[e] for the
-// coefficients with n + m even, e = ev_index(n, m) (the others vanish for an
-// in-plane shift). Returns false when every Gamma(1/2-j, z) underflows.
+// The terms of one K point times `phase` (e^{i K.s}, or 1): emit(e, t) for
+// the coefficients with n + m even, e = ev_index(n, m) (the others vanish for
+// an in-plane shift), so a caller that accumulates needs no array of them.
+// The m-independent products (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, z)
+// are formed once per (n, j), and the phase is folded into e^{i m alpha}.
+// Returns false when every Gamma(1/2-j, z) underflows.
 template <int LE> constexpr int ev_count() { return (LE + 1) * (LE + 2) / 2; }
 __host__ __device__ constexpr int ev_index(int n, int m) { return n * (n + 1) / 2 + (m + n) / 2; }
 
-template <int LE>
+template <int LE, class Emit>
 __device__ __forceinline__ bool long_point(const double *__restrict__ T, double Kx, double Ky, double eta,
                                            const double *__restrict__ leg0, const v2d *__restrict__ s1,
-                                           const int *__restrict__ s1off, cplx (&f)[ev_count<LE>()]) {
+                                           const int *__restrict__ s1off, cplx phase, Emit emit) {
   constexpr int JM = LE / 2;
   const cplx k = mk(T[3], T[4]);
   const bool kreal = (T[4] == 0.0);
@@ -180,26 +183,37 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
 #pragma unroll
     for (int j = 0; j <= JM; ++j) G[j] = qew::gamma_inc_half(j, z, branch);
   }
-  // gamma^{2j-1}
-  cplx gp[JM + 1];
-  gp[0] = cinv(gam);
-  const cplx g2 = gam * gam;
+  // gamma^{2j-1} Gamma(1/2-j, z)
+  cplx gG[JM + 1];
+  {
+    const cplx g2 = gam * gam;
+    cplx gp = cinv(gam);
 #pragma unroll
-  for (int j = 1; j <= JM; ++j) gp[j] = (j == 1) ? gam : gp[j - 1] * g2;
+    for (int j = 0; j <= JM; ++j) {
+      gG[j] = gp * G[j];
+      gp = (j == 0) ? gam : gp * g2;
+    }
+  }
   // (|beta+K|/k)^p, p = 0..LE
   const cplx rk = kreal ? mk(rb / k.re, 0.0) : mk(rb, 0.0) / k;
   cplx rkp[LE + 1];
   rkp[0] = mk(1.0, 0.0);
 #pragma unroll
   for (int p = 1; p <= LE; ++p) rkp[p] = rkp[p - 1] * rk;
-  // e^{i alpha}, alpha = atan2(by, bx) (0 where beta + K vanishes)
+  // e^{i alpha} (alpha = atan2(by, bx), 0 where beta + K vanishes); per n
+  // the factors phase e^{i m alpha}, m = -n, -n+2, ..., n, from
+  // phase e^{-i n alpha} by steps of e^{2 i alpha} (three complex values live
+  // instead of two tables of LE + 1)
   const cplx e1 = (rb != 0.0) ? mk(bx / rb, by / rb) : mk(1.0, 0.0);
-  cplx em[LE + 1];
-  em[0] = mk(1.0, 0.0);
-#pragma unroll
-  for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
+  const cplx e2 = e1 * e1, e1c = conj(e1);
+  cplx base = phase;   // phase e^{-i n alpha}
 #pragma unroll
   for (int n = 0; n <= LE; ++n) {
+    if (n > 0) base = base * e1c;
+    cplx em = base;
+    cplx P[JM + 1];   // (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, z)
+#pragma unroll
+    for (int j = 0; j <= n / 2; ++j) P[j] = rkp[n - 2 * j] * gG[j];
 #pragma unroll
     for (int m = -n; m <= n; m += 2) {
       const int am = cabsi(m);
@@ -209,12 +223,12 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
 #pragma unroll
       for (int j = 0; j <= (n - am) / 2; ++j) {
         const v2d c = s1[off + j];
-        js = js + ((rkp[n - 2 * j] * gp[j]) * G[j]) * mk(c.x, c.y);
+        js = js + P[j] * mk(c.x, c.y);
       }
-      const cplx e = (m >= 0) ? em[am] : conj(em[am]);
-      cplx t = (leg0[n * (n + 1) / 2 + am] * js) * e;
+      cplx t = (leg0[n * (n + 1) / 2 + am] * js) * em;
       if (m < 0 && (am & 1)) t = -t;
-      f[ev_index(n, m)] = t;
+      emit(ev_index(n, m), t);
+      em = em * e2;
     }
   }
   return true;
@@ -240,34 +254,30 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
   const double *T = tr + 8 * (size_t)t;
   const cplx k = mk(T[3], T[4]);
   const bool kreal = (T[4] == 0.0);
-  cplx acc[NSC];
+  constexpr int NE = ev_count<LE>();
+  cplx acc[NE];   // the coefficients with n + m even (ev_index); the others vanish
 #pragma unroll
-  for (int i = 0; i < NSC; ++i) acc[i] = mk(0.0, 0.0);
+  for (int i = 0; i < NE; ++i) acc[i] = mk(0.0, 0.0);
   double aerr = 0.0;
   for (int ip = lane; ip < npts; ip += 64) {
     const double Kx = Kp[2 * ip], Ky = Kp[2 * ip + 1];
-    cplx f[ev_count<LE>()];
-    if (!long_point<LE>(T, Kx, Ky, eta, leg0, s1, s1off, f)) continue;
-    const cplx phase = cexpi(Kx * T[5] + Ky * T[6]);
-#pragma unroll
-    for (int n = 0; n <= LE; ++n) {
-#pragma unroll
-      for (int m = -n; m <= n; m += 2) {
-        const cplx v = f[ev_index(n, m)] * phase;
-        acc[n * (n + 1) + m] = acc[n * (n + 1) + m] + v;
-        aerr += fabs(v.re) + fabs(v.im);
-      }
-    }
+    long_point<LE>(T, Kx, Ky, eta, leg0, s1, s1off, cexpi(Kx * T[5] + Ky * T[6]), [&](int e, cplx v) {
+      acc[e] = acc[e] + v;
+      aerr += fabs(v.re) + fabs(v.im);
+    });
   }
   const cplx cf = kreal ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
 #pragma unroll
-  for (int i = 0; i < NSC; ++i) {
-    const double re = wave_sum(acc[i].re), im = wave_sum(acc[i].im);
-    if (lane == 0) {
-      const cplx v = mk(re, im) * cf;
-      out[(size_t)t * NSC + i] = v2d{v.re, v.im};
+  for (int n = 0; n <= LE; ++n)
+#pragma unroll
+    for (int m = -n; m <= n; ++m) {
+      cplx v = mk(0.0, 0.0);
+      if (!((n + m) & 1)) {
+        const int e = ev_index(n, m);
+        v = mk(wave_sum(acc[e].re), wave_sum(acc[e].im)) * cf;
+      }
+      if (lane == 0) out[(size_t)t * NSC + n * (n + 1) + m] = v2d{v.re, v.im};
     }
-  }
   aerr = wave_sum(aerr);
   if (err && lane == 0)
     for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr * cabsd(cf);
@@ -289,7 +299,8 @@ __global__ __launch_bounds__(64) void k_long_coef(const double *__restrict__ T, 
   const cplx k = mk(T[3], T[4]);
   const cplx cf = (T[4] == 0.0) ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
   cplx f[NE];
-  const bool live = long_point<LE>(T, Kp[2 * ip], Kp[2 * ip + 1], eta, leg0, s1, s1off, f);
+  const bool live = long_point<LE>(T, Kp[2 * ip], Kp[2 * ip + 1], eta, leg0, s1, s1off, mk(1.0, 0.0),
+                                   [&](int e, cplx v) { f[e] = v; });
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const cplx v = live ? f[e] * cf : mk(0.0, 0.0);
