@@ -18,8 +18,7 @@
 int qh_launch_sigma_short(int LE, int xy, int ntrip, const double *tr_dev, int npts, const double *pts_dev,
 		double eta, const double *leg0_dev, int csphase, double *out_dev, double *err_dev, void *stream);
 int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev,
-		double eta, double area, const double *leg0_dev, const double *s1_dev,
-		const int *s1off_dev, double *out_dev, double *err_dev, void *stream);
+		double eta, double area, const double *leg0_dev, double *out_dev, double *err_dev, void *stream);
 int qh_launch_sigma0(int n, const double *k_dev, double eta, double leg00, double *out_dev, void *stream);
 int qh_launch_sigma_total(int ntrip, int nsc, const double *ss, const double *sl, const double *s0,
 		const int *flag0, int parts, double *tot, void *stream);
@@ -32,11 +31,12 @@ void qh_trans_array_from_AB(_Complex double *t, const qpms_vswf_set_spec_t *dest
 /* ---------------------------------------------------------------------- */
 /* Device copies of an Ewald constant table (keyed by the table address)    */
 /* ---------------------------------------------------------------------- */
+/* The long-range kernels form the s1 constants (ewald.c:141-145) from their
+ * closed form at compile time; only P_n^m(0), which carries the table's
+ * Condon-Shortley phase, is read from the table. */
 typedef struct qh_e3dev {
 	const qpms_ewald3_constants_t *c;
 	double *leg0;     /* P_n^m(0), n <= lMax */
-	double *s1;       /* complex constants, packed */
-	int *s1off;       /* per y_sc offset into s1 */
 	struct qh_e3dev *next;
 } qh_e3dev;
 static qh_e3dev *qh_e3reg = NULL;
@@ -50,23 +50,10 @@ static const qh_e3dev *qh_e3_get(const qpms_ewald3_constants_t *c) {
 	QH_CALLOC(e, 1, sizeof(*e));
 	e->c = c;
 	const int L = c->lMax;
-	const size_t nleg = (size_t)(L + 1) * (L + 2) / 2, nsc = c->nelem_sc;
-	size_t ns1 = 0;
-	int *off;
-	QH_MALLOC(off, nsc * sizeof(int));
-	for (size_t y = 0; y < nsc; ++y) { off[y] = (int)ns1; ns1 += (c->s1_jMaxes[y] >= 0) ? (size_t)c->s1_jMaxes[y] + 1 : 0; }
-	complex double *s1;
-	QH_MALLOC(s1, (ns1 ? ns1 : 1) * sizeof(complex double));
-	for (size_t y = 0; y < nsc; ++y)
-		for (int j = 0; j <= c->s1_jMaxes[y]; ++j) s1[off[y] + j] = c->s1_constfacs[y][j];
+	const size_t nleg = (size_t)(L + 1) * (L + 2) / 2;
 	e->leg0 = qh_dmalloc(nleg * sizeof(double));
-	e->s1 = qh_dmalloc((ns1 ? ns1 : 1) * sizeof(complex double));
-	e->s1off = qh_dmalloc(nsc * sizeof(int));
 	qh_h2d(e->leg0, c->legendre0, nleg * sizeof(double), NULL);
-	qh_h2d(e->s1, s1, ns1 * sizeof(complex double), NULL);
-	qh_h2d(e->s1off, off, nsc * sizeof(int), NULL);
 	qh_sync(NULL);
-	free(s1); free(off);
 	e->next = qh_e3reg;
 	qh_e3reg = e;
 	pthread_mutex_unlock(&qh_e3lock);
@@ -81,7 +68,7 @@ void qh_e3_release(const qpms_ewald3_constants_t *c) {
 		if ((*pp)->c == c) {
 			qh_e3dev *e = *pp;
 			*pp = e->next;
-			qh_dfree(e->leg0); qh_dfree(e->s1); qh_dfree(e->s1off);
+			qh_dfree(e->leg0);
 			free(e);
 			break;
 		}
@@ -105,8 +92,8 @@ int qpms_hip_ewald3_sigma_batch_dev(const qpms_ewald3_constants_t *c, double eta
 		qh_launch_sigma_short(c->lMax, 1, (int)ntrip, triples_dev, (int)nR, R_dev, eta, e->leg0,
 			c->legendre_csphase, (double *)sigma_short_dev, NULL, stream);
 	if (sigma_long_dev)
-		qh_launch_sigma_long(c->lMax, (int)ntrip, triples_dev, (int)nK, K_dev, eta, unitcell_area, e->leg0, e->s1,
-			e->s1off, (double *)sigma_long_dev, NULL, stream);
+		qh_launch_sigma_long(c->lMax, (int)ntrip, triples_dev, (int)nK, K_dev, eta, unitcell_area, e->leg0,
+			(double *)sigma_long_dev, NULL, stream);
 	return 0;
 }
 
@@ -115,8 +102,7 @@ int qpms_hip_ewald3_sigma_batch_dev(const qpms_ewald3_constants_t *c, double eta
  * nK) bytes of device scratch. */
 size_t qh_sigma_long_shifts_scratch(int LE, int npts);
 int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev, double eta,
-		double area, const double *leg0_dev, const double *s1_dev, const int *s1off_dev, double *F_dev,
-		double *out_dev, void *stream);
+		double area, const double *leg0_dev, double *F_dev, double *out_dev, void *stream);
 size_t qh_ewald3_long_shifts_scratch(const qpms_ewald3_constants_t *c, size_t nK) {
 	return qh_sigma_long_shifts_scratch(c->lMax, (int)nK);
 }
@@ -126,7 +112,7 @@ int qh_ewald3_sigma_long_shifts_dev(const qpms_ewald3_constants_t *c, double eta
 	qh_check_le(c->lMax);
 	const qh_e3dev *e = qh_e3_get(c);
 	return qh_launch_sigma_long_shifts(c->lMax, (int)ntrip, triples_dev, (int)nK, K_dev, eta, unitcell_area, e->leg0,
-		e->s1, e->s1off, (double *)F_dev, (double *)sigma_long_dev, stream);
+		(double *)F_dev, (double *)sigma_long_dev, stream);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -223,7 +209,7 @@ int ewald3_sigma_long(complex double *target, double *err, const qpms_ewald3_con
 	qh_h2d(dtr, tr, 64, NULL);
 	qh_h2d(dK, K, np * 16, NULL);
 	const qh_e3dev *e = qh_e3_get(c);
-	qh_launch_sigma_long(c->lMax, 1, dtr, (int)np, dK, eta, unitcell_volume, e->leg0, e->s1, e->s1off, dout, derr, NULL);
+	qh_launch_sigma_long(c->lMax, 1, dtr, (int)np, dK, eta, unitcell_volume, e->leg0, dout, derr, NULL);
 	qh_d2h(target, dout, nsc * 16, NULL);
 	if (err) qh_d2h(err, derr, nsc * 8, NULL);
 	qh_sync(NULL);
@@ -293,7 +279,7 @@ static void qh_e32_device(const qpms_trans_calculator *c, int ntrip, const cart3
 	qh_h2d(dflag, flag0, ntrip * sizeof(int), NULL);
 	const qh_e3dev *e = qh_e3_get(e3);
 	if (parts & QPMS_EWALD_LONG_RANGE)
-		qh_launch_sigma_long(e3->lMax, ntrip, dtr, (int)nK, dK, eta, area, e->leg0, e->s1, e->s1off, dsl, NULL, NULL);
+		qh_launch_sigma_long(e3->lMax, ntrip, dtr, (int)nK, dK, eta, area, e->leg0, dsl, NULL, NULL);
 	/* one launch for every triple: the kernel drops the lattice point equal
 	 * to the shift, i.e. the origin for s = 0 */
 	if (parts & QPMS_EWALD_SHORT_RANGE)

@@ -38,24 +38,33 @@ __device__ __forceinline__ double wave_sum(double v) {
 //   sigma^S_{n,m} = sum_R -i (2/k)^{n+1}/sqrt(pi) |R-s|^n P_n^{|m|}(cos th)
 //                   I_n(|R-s|) e^{i beta.(R-s)} e^{i m phi} (-1)^m [m < 0]
 // pts: lattice points (x, y, z); the triple's shift s is subtracted.
-// XY: all points and s in the plane z = 0, P_n^{|m|}(0) from leg0 and the
-// odd n+m coefficients vanish.
+// XY: all points and s in the plane z = 0: P_n^{|m|}(0) from leg0 and the
+// odd n+m coefficients vanish. The factors that do not depend on the point,
+// -i (2/k)^{n+1}/sqrt(pi), P_n^{|m|}(0) and the sign of odd negative m, are
+// then applied to the sums, and only the (LE+1)(LE+2)/2 coefficients with
+// n + m even are accumulated (ev_index).
+// ERR: also the error estimate (2^-52 times the sum of |Re| + |Im| of the
+// terms).
 // ---------------------------------------------------------------------------
-template <int LE, bool XY>
-__global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__restrict__ tr, int npts,
-                                                     const double *__restrict__ pts, double eta,
-                                                     const double *__restrict__ leg0, int csphase,
-                                                     v2d *__restrict__ out, double *__restrict__ err) {
+template <int LE> constexpr int ev_count() { return (LE + 1) * (LE + 2) / 2; }
+__host__ __device__ constexpr int ev_index(int n, int m) { return n * (n + 1) / 2 + (m + n) / 2; }
+
+template <int LE, bool XY, bool ERR>
+__global__ __launch_bounds__(64, (XY && !ERR && LE <= 7) ? 2 : 1) void k_sigma_short(
+    int ntrip, const double *__restrict__ tr, int npts, const double *__restrict__ pts, double eta,
+    const double *__restrict__ leg0, int csphase, v2d *__restrict__ out, double *__restrict__ err) {
   constexpr int NSC = (LE + 1) * (LE + 1);
+  constexpr int NA = XY ? ev_count<LE>() : NSC;
   const int t = blockIdx.x;
   if (t >= ntrip) return;
   const int lane = threadIdx.x;
   const double *T = tr + 8 * (size_t)t;  // bx by bz kre kim sx sy sz
   const cplx k = mk(T[3], T[4]);
   const bool kreal = (T[4] == 0.0);
-  cplx pre[LE + 1];  // -i (2/k)^{n+1} / sqrt(pi)
-  {
-    const cplx tk = kreal ? mk(2.0 / T[3], 0.0) : 2.0 * cinv(k);
+  // -i (2/k)^{n+1} / sqrt(pi): per term (3D shifts) or on the sums (XY)
+  const cplx tk = kreal ? mk(2.0 / T[3], 0.0) : 2.0 * cinv(k);
+  cplx pre[XY ? 1 : LE + 1];
+  if constexpr (!XY) {
     cplx p = tk;
 #pragma unroll
     for (int n = 0; n <= LE; ++n) {
@@ -63,9 +72,9 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
       p = p * tk;
     }
   }
-  cplx acc[NSC];
+  cplx acc[NA];
 #pragma unroll
-  for (int i = 0; i < NSC; ++i) acc[i] = mk(0.0, 0.0);
+  for (int i = 0; i < NA; ++i) acc[i] = mk(0.0, 0.0);
   double aerr = 0.0;
   // Every short-range term carries exp(-R^2 eta^2 + Re kappa^2), kappa = k / (2 eta):
   // where that exponent is below -746 the factor is exactly 0 in double and
@@ -84,43 +93,78 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
     // e^{i phi}, phi = atan2(ry, rx); on the z axis only m = 0 terms survive
     const double rho = sqrt(rx * rx + ry * ry);
     const cplx e1 = (rho != 0.0) ? mk(rx / rho, ry / rho) : mk(1.0, 0.0);
-    double P[XY ? 1 : qsf::leg_n(LE)];
-    if (!XY) {
+    if constexpr (XY) {
+      // per n: b = |R-s|^n I_n e^{i beta.(R-s)} e^{-i n phi}, stepped by e^{2 i phi}
+      const cplx e2 = e1 * e1, e1c = conj(e1);
+      cplx ph = eb;   // e^{i beta.(R-s)} e^{-i n phi}
+      double rn = 1.0, tn = 0.56418958354775628695 * cabsd(tk);   // |-i (2/k)^{n+1} / sqrt(pi)|
+#pragma unroll
+      for (int n = 0; n <= LE; ++n) {
+        if (n > 0) ph = ph * e1c;
+        cplx b = (rn * In[n]) * ph;
+#pragma unroll
+        for (int m = -n; m <= n; m += 2) {
+          const int e = ev_index(n, m);
+          acc[e] = acc[e] + b;
+          if (ERR) aerr += fabs(leg0[n * (n + 1) / 2 + cabsi(m)]) * tn * (fabs(b.re) + fabs(b.im));
+          b = b * e2;
+        }
+        rn *= r;
+        if (ERR) tn *= cabsd(tk);
+      }
+    } else {
+      double P[qsf::leg_n(LE)];
       const double x = rz / r;
       qsf::legendre_cs<LE>(x, sqrt((1.0 - x) * (1.0 + x)), P);
-    }
-    cplx em[LE + 1];
-    em[0] = mk(1.0, 0.0);
+      cplx em[LE + 1];
+      em[0] = mk(1.0, 0.0);
 #pragma unroll
-    for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
-    double rn = 1.0;
+      for (int m = 1; m <= LE; ++m) em[m] = em[m - 1] * e1;
+      double rn = 1.0;
 #pragma unroll
-    for (int n = 0; n <= LE; ++n) {
-      const cplx base = (pre[n] * (rn * In[n])) * eb;
+      for (int n = 0; n <= LE; ++n) {
+        const cplx base = (pre[n] * (rn * In[n])) * eb;
 #pragma unroll
-      for (int m = -n; m <= n; ++m) {
-        if (XY && ((n + m) & 1)) continue;
-        const int am = cabsi(m);
-        double leg;
-        if (XY) leg = leg0[n * (n + 1) / 2 + am];
-        else leg = ((csphase == 1) && (am & 1)) ? -P[n * (n + 1) / 2 + am] : P[n * (n + 1) / 2 + am];
-        const cplx e = (m >= 0) ? em[am] : conj(em[am]);
-        cplx s = (leg * base) * e;
-        if (m < 0 && (am & 1)) s = -s;
-        acc[n * (n + 1) + m] = acc[n * (n + 1) + m] + s;
-        aerr += fabs(s.re) + fabs(s.im);
+        for (int m = -n; m <= n; ++m) {
+          const int am = cabsi(m);
+          const double leg = ((csphase == 1) && (am & 1)) ? -P[n * (n + 1) / 2 + am] : P[n * (n + 1) / 2 + am];
+          const cplx e = (m >= 0) ? em[am] : conj(em[am]);
+          cplx s = (leg * base) * e;
+          if (m < 0 && (am & 1)) s = -s;
+          acc[n * (n + 1) + m] = acc[n * (n + 1) + m] + s;
+          if (ERR) aerr += fabs(s.re) + fabs(s.im);
+        }
+        rn *= r;
       }
-      rn *= r;
     }
   }
+  cplx p = tk;   // (2/k)^{n+1}
 #pragma unroll
-  for (int i = 0; i < NSC; ++i) {
-    const double re = wave_sum(acc[i].re), im = wave_sum(acc[i].im);
-    if (lane == 0) out[(size_t)t * NSC + i] = v2d{re, im};
+  for (int n = 0; n <= LE; ++n) {
+#pragma unroll
+    for (int m = -n; m <= n; ++m) {
+      cplx v;
+      if constexpr (XY) {
+        v = mk(0.0, 0.0);
+        if (!((n + m) & 1)) {
+          const int am = cabsi(m);
+          const int e = ev_index(n, m);
+          const double f = (m < 0 && (am & 1)) ? -leg0[n * (n + 1) / 2 + am] : leg0[n * (n + 1) / 2 + am];
+          v = mk(wave_sum(acc[e].re), wave_sum(acc[e].im)) * (mk(p.im, -p.re) * (0.56418958354775628695 * f));
+        }
+      } else {
+        const int i = n * (n + 1) + m;
+        v = mk(wave_sum(acc[i].re), wave_sum(acc[i].im));
+      }
+      if (lane == 0) out[(size_t)t * NSC + n * (n + 1) + m] = v2d{v.re, v.im};
+    }
+    p = p * tk;
   }
-  aerr = wave_sum(aerr);
-  if (err && lane == 0)
-    for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr;
+  if (ERR) {
+    aerr = wave_sum(aerr);
+    if (lane == 0)
+      for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -128,21 +172,52 @@ __global__ __launch_bounds__(64) void k_sigma_short(int ntrip, const double *__r
 //   sigma^L_{n,m} = 1/(k^2 A) sum_K e^{i K.s} e^{i m alpha} P_n^{|m|}(0) (-1)^m[m<0]
 //     sum_j (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, x^2) c_{n,m,j},
 //   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
-// s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
+// c_{n,m,j} = -2 i^{n+1} sqrt(pi) q_{n,|m|,j} (ewald.c:141-145, the s1
+// constants), q = (-1)^j a! b! / (j! (a-j)! (b-j)!), a, b = (n -+ |m|)/2: an
+// integer. P_n^{|m|}(0) is a dyadic rational, so P_n^{|m|}(0) q_{n,|m|,j} is an
+// exact double formed at compile time and the per-term arithmetic has no table
+// loads. The factor that remains per coefficient,
+//   leg0[n,|m|] / P_n^{|m|}(0) * (-2 sqrt(pi)) i^{n+1}
+// (leg0 carries the Condon-Shortley phase the caller chose), is applied to the
+// sums (long_out_factor).
 // ---------------------------------------------------------------------------
-// The terms of one K point times `phase` (e^{i K.s}, or 1): emit(e, t) for
-// the coefficients with n + m even, e = ev_index(n, m) (the others vanish for
-// an in-plane shift), so a caller that accumulates needs no array of them.
-// The m-independent products (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, z)
-// are formed once per (n, j), and the phase is folded into e^{i m alpha}.
-// Returns false when every Gamma(1/2-j, z) underflows.
-template <int LE> constexpr int ev_count() { return (LE + 1) * (LE + 2) / 2; }
-__host__ __device__ constexpr int ev_index(int n, int m) { return n * (n + 1) / 2 + (m + n) / 2; }
+// P_n^{am}(0) without the Condon-Shortley phase (n + am even):
+// (-1)^{(n-am)/2} (n+am-1)!! / (n-am)!!
+constexpr double ferrers_zero(int n, int am) {
+  double num = 1.0, den = 1.0;
+  for (int i = n + am - 1; i > 1; i -= 2) num *= i;
+  for (int i = n - am; i > 1; i -= 2) den *= i;
+  return (((n - am) / 2) & 1) ? -num / den : num / den;
+}
+// P_n^{am}(0) q_{n,am,j}
+constexpr double long_coef_q(int n, int am, int j) {
+  const int a = (n - am) / 2, b = (n + am) / 2;
+  double f = 1.0;
+  for (int i = 0; i < j; ++i) f *= (double)(a - i);              // a! / (a-j)!
+  for (int i = 0; i < j; ++i) f = f * (double)(b - i) / (i + 1);  // times C(b, j), exact at each step
+  return ferrers_zero(n, am) * ((j & 1) ? -f : f);
+}
+__device__ __forceinline__ cplx long_out_factor(int n, int am, const double *__restrict__ leg0) {
+  const double r = leg0[n * (n + 1) / 2 + am] / ferrers_zero(n, am) * (-2.0 * 1.7724538509055160273);
+  switch ((n + 1) & 3) {   // i^{n+1}
+    case 0: return mk(r, 0.0);
+    case 1: return mk(0.0, r);
+    case 2: return mk(-r, 0.0);
+    default: return mk(0.0, -r);
+  }
+}
 
+// The terms of one K point times `phase` (e^{i K.s}, or 1), each without its
+// coefficient's long_out_factor: emit(e, t) for the coefficients with n + m
+// even, e = ev_index(n, m) (the others vanish for an in-plane shift), so a
+// caller that accumulates needs no array of them. The m-independent products
+// (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, z) are formed once per (n, j),
+// the j sums once per (n, |m|) (the sum of -m is that of m, negated for odd
+// |m|), and the phase is folded into e^{i m alpha}.
+// Returns false when every Gamma(1/2-j, z) underflows.
 template <int LE, class Emit>
 __device__ __forceinline__ bool long_point(const double *__restrict__ T, double Kx, double Ky, double eta,
-                                           const double *__restrict__ leg0, const v2d *__restrict__ s1,
-                                           const int *__restrict__ s1off, cplx phase, Emit emit) {
+                                           cplx phase, Emit emit) {
   constexpr int JM = LE / 2;
   const cplx k = mk(T[3], T[4]);
   const bool kreal = (T[4] == 0.0);
@@ -210,23 +285,22 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
 #pragma unroll
   for (int n = 0; n <= LE; ++n) {
     if (n > 0) base = base * e1c;
-    cplx em = base;
     cplx P[JM + 1];   // (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, z)
 #pragma unroll
     for (int j = 0; j <= n / 2; ++j) P[j] = rkp[n - 2 * j] * gG[j];
+    cplx js[LE + 1];  // per |m| (n + |m| even): sum_j P_n^{|m|}(0) q_{n,|m|,j} P[j]
+#pragma unroll
+    for (int am = n & 1; am <= n; am += 2) {
+      cplx a = P[0] * long_coef_q(n, am, 0);
+#pragma unroll
+      for (int j = 1; j <= (n - am) / 2; ++j) a = a + P[j] * long_coef_q(n, am, j);
+      js[am] = a;
+    }
+    cplx em = base;
 #pragma unroll
     for (int m = -n; m <= n; m += 2) {
       const int am = cabsi(m);
-      const int y = n * (n + 1) + m;
-      cplx js = mk(0.0, 0.0);
-      const int off = s1off[y];
-#pragma unroll
-      for (int j = 0; j <= (n - am) / 2; ++j) {
-        const v2d c = s1[off + j];
-        js = js + P[j] * mk(c.x, c.y);
-      }
-      cplx t = (leg0[n * (n + 1) / 2 + am] * js) * em;
-      if (m < 0 && (am & 1)) t = -t;
+      const cplx t = (m < 0 && (am & 1)) ? -(js[am] * em) : js[am] * em;
       emit(ev_index(n, m), t);
       em = em * e2;
     }
@@ -239,13 +313,12 @@ __device__ __forceinline__ bool long_point(const double *__restrict__ T, double 
 //   sigma^L_{n,m} = 1/(k^2 A) sum_K e^{i K.s} e^{i m alpha} P_n^{|m|}(0) (-1)^m[m<0]
 //     sum_j (|beta+K|/k)^{n-2j} gamma^{2j-1} Gamma(1/2-j, x^2) c_{n,m,j},
 //   gamma = clilgamma(|beta+K|/k), x = gamma k / (2 eta).
-// s1: per (n,m) with n+m even, (n-|m|)/2 + 1 constants at s1off[y].
+// One wave per triple, lanes over K points; ERR: also the error estimate.
 // ---------------------------------------------------------------------------
-template <int LE>
-__global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__restrict__ tr, int npts,
+template <int LE, bool ERR>
+__global__ __launch_bounds__(64, LE <= 7 ? 2 : 1) void k_sigma_long(int ntrip, const double *__restrict__ tr, int npts,
                                                     const double *__restrict__ Kp /* x, y */, double eta,
                                                     double area, const double *__restrict__ leg0,
-                                                    const v2d *__restrict__ s1, const int *__restrict__ s1off,
                                                     v2d *__restrict__ out, double *__restrict__ err) {
   constexpr int NSC = (LE + 1) * (LE + 1);
   const int t = blockIdx.x;
@@ -261,9 +334,9 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
   double aerr = 0.0;
   for (int ip = lane; ip < npts; ip += 64) {
     const double Kx = Kp[2 * ip], Ky = Kp[2 * ip + 1];
-    long_point<LE>(T, Kx, Ky, eta, leg0, s1, s1off, cexpi(Kx * T[5] + Ky * T[6]), [&](int e, cplx v) {
+    long_point<LE>(T, Kx, Ky, eta, cexpi(Kx * T[5] + Ky * T[6]), [&](int e, cplx v) {
       acc[e] = acc[e] + v;
-      aerr += fabs(v.re) + fabs(v.im);
+      if (ERR) aerr += fabs(v.re) + fabs(v.im);
     });
   }
   const cplx cf = kreal ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
@@ -274,13 +347,17 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
       cplx v = mk(0.0, 0.0);
       if (!((n + m) & 1)) {
         const int e = ev_index(n, m);
-        v = mk(wave_sum(acc[e].re), wave_sum(acc[e].im)) * cf;
+        v = mk(wave_sum(acc[e].re), wave_sum(acc[e].im)) * (cf * long_out_factor(n, cabsi(m), leg0));
       }
       if (lane == 0) out[(size_t)t * NSC + n * (n + 1) + m] = v2d{v.re, v.im};
     }
-  aerr = wave_sum(aerr);
-  if (err && lane == 0)
-    for (int i = 0; i < NSC; ++i) err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr * cabsd(cf);
+  if (ERR) {
+    // every out-factor has modulus 2 sqrt(pi) |leg0 / P(0)| = 2 sqrt(pi)
+    aerr = wave_sum(aerr);
+    if (lane == 0)
+      for (int i = 0; i < NSC; ++i)
+        err[(size_t)t * NSC + i] = 2.2204460492503131e-16 * aerr * (2.0 * 1.7724538509055160273) * cabsd(cf);
+  }
 }
 
 // Long range for many shifts sharing (beta, k), as two passes: the terms of
@@ -291,21 +368,23 @@ __global__ __launch_bounds__(64) void k_sigma_long(int ntrip, const double *__re
 template <int LE>
 __global__ __launch_bounds__(64) void k_long_coef(const double *__restrict__ T, int npts,
                                                    const double *__restrict__ Kp, double eta, double area,
-                                                   const double *__restrict__ leg0, const v2d *__restrict__ s1,
-                                                   const int *__restrict__ s1off, v2d *__restrict__ F) {
+                                                   const double *__restrict__ leg0, v2d *__restrict__ F) {
   constexpr int NE = ev_count<LE>();
   const int ip = blockIdx.x * 64 + threadIdx.x;
   if (ip >= npts) return;
   const cplx k = mk(T[3], T[4]);
   const cplx cf = (T[4] == 0.0) ? mk(1.0 / (k.re * k.re * area), 0.0) : cinv((k * k) * area);
   cplx f[NE];
-  const bool live = long_point<LE>(T, Kp[2 * ip], Kp[2 * ip + 1], eta, leg0, s1, s1off, mk(1.0, 0.0),
+  const bool live = long_point<LE>(T, Kp[2 * ip], Kp[2 * ip + 1], eta, mk(1.0, 0.0),
                                    [&](int e, cplx v) { f[e] = v; });
 #pragma unroll
-  for (int e = 0; e < NE; ++e) {
-    const cplx v = live ? f[e] * cf : mk(0.0, 0.0);
-    F[(size_t)ip * NE + e] = v2d{v.re, v.im};
-  }
+  for (int n = 0; n <= LE; ++n)
+#pragma unroll
+    for (int m = -n; m <= n; m += 2) {
+      const int e = ev_index(n, m);
+      const cplx v = live ? f[e] * (cf * long_out_factor(n, cabsi(m), leg0)) : mk(0.0, 0.0);
+      F[(size_t)ip * NE + e] = v2d{v.re, v.im};
+    }
 }
 
 constexpr int LS_KT = 16;   // K points per LDS tile
@@ -456,27 +535,32 @@ extern "C" int qh_launch_sigma_short(int LE, int xy, int ntrip, const double *tr
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
   const dim3 g(ntrip), b(64);
-#define QH_SS(L) case L: if (xy) QH_LAUNCH((k_sigma_short<L, true>), g, b, 0, s, ntrip, tr_dev, npts, pts_dev, eta, \
-      leg0_dev, csphase, (v2d *)out_dev, err_dev); else QH_LAUNCH((k_sigma_short<L, false>), g, b, 0, s, ntrip, tr_dev, \
-      npts, pts_dev, eta, leg0_dev, csphase, (v2d *)out_dev, err_dev); break;
+#define QH_SS1(L, X, E) QH_LAUNCH((k_sigma_short<L, X, E>), g, b, 0, s, ntrip, tr_dev, npts, pts_dev, eta, leg0_dev, \
+      csphase, (v2d *)out_dev, err_dev)
+#define QH_SS(L) case L: \
+    if (xy) { if (err_dev) QH_SS1(L, true, true); else QH_SS1(L, true, false); } \
+    else { if (err_dev) QH_SS1(L, false, true); else QH_SS1(L, false, false); } \
+    break;
   switch (LE) {
     QH_SS(3) QH_SS(5) QH_SS(7) QH_SS(9)
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
   }
 #undef QH_SS
+#undef QH_SS1
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 extern "C" int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev,
-                                    double eta, double area, const double *leg0_dev, const double *s1_dev,
-                                    const int *s1off_dev, double *out_dev, double *err_dev, void *stream) {
+                                    double eta, double area, const double *leg0_dev, double *out_dev,
+                                    double *err_dev, void *stream) {
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
   ensure_weid();   // the evanescent orders use the erfcx expansion
   const dim3 g(ntrip), b(64);
-#define QH_SL(L) case L: QH_LAUNCH(k_sigma_long<L>, g, b, 0, s, ntrip, tr_dev, npts, K_dev, eta, area, leg0_dev, \
-      (const v2d *)s1_dev, s1off_dev, (v2d *)out_dev, err_dev); break;
+#define QH_SL(L) case L: if (err_dev) QH_LAUNCH((k_sigma_long<L, true>), g, b, 0, s, ntrip, tr_dev, npts, K_dev, eta, \
+      area, leg0_dev, (v2d *)out_dev, err_dev); else QH_LAUNCH((k_sigma_long<L, false>), g, b, 0, s, ntrip, tr_dev, \
+      npts, K_dev, eta, area, leg0_dev, (v2d *)out_dev, err_dev); break;
   switch (LE) {
     QH_SL(3) QH_SL(5) QH_SL(7) QH_SL(9)
     default: qpms_hip_die(__FILE__, __LINE__, __func__, "Ewald lMax %d not instantiated (3, 5, 7, 9)", LE);
@@ -490,15 +574,15 @@ extern "C" int qh_launch_sigma_long(int LE, int ntrip, const double *tr_dev, int
 // F: scratch of npts * (LE+1)(LE+2)/2 complex values.
 extern "C" size_t qh_sigma_long_shifts_scratch(int LE, int npts) { return (size_t)npts * (LE + 1) * (LE + 2) / 2 * 16; }
 extern "C" int qh_launch_sigma_long_shifts(int LE, int ntrip, const double *tr_dev, int npts, const double *K_dev,
-                                           double eta, double area, const double *leg0_dev, const double *s1_dev,
-                                           const int *s1off_dev, double *F_dev, double *out_dev, void *stream) {
+                                           double eta, double area, const double *leg0_dev, double *F_dev,
+                                           double *out_dev, void *stream) {
   if (ntrip <= 0) return 0;
   hipStream_t s = (hipStream_t)qh_stream(stream);
   ensure_weid();   // the evanescent orders use the erfcx expansion
   const dim3 gc((npts + 63) / 64), gs((ntrip + 255) / 256);
 #define QH_LS(L) case L: \
     if (npts > 0) QH_LAUNCH(k_long_coef<L>, gc, dim3(64), 0, s, tr_dev, npts, K_dev, eta, area, leg0_dev, \
-                                     (const v2d *)s1_dev, s1off_dev, (v2d *)F_dev); \
+                            (v2d *)F_dev); \
     QH_LAUNCH(k_long_shifts<L>, gs, dim3(256), 0, s, ntrip, tr_dev, npts, K_dev, (const v2d *)F_dev, \
                        (v2d *)out_dev); break;
   switch (LE) {
