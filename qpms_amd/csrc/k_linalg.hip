@@ -1230,6 +1230,99 @@ static bool use_lookahead() {
   return v == 1;
 }
 
+// C -= A B (or C = A B, ASSIGN) for K <= PW: the strip's products (L21 =
+// A21 U11^{-1}, U = L11^{-1} A12, the in-block rank-PW update) when they run
+// as whole-CU workgroups beside the trailing update (one workgroup per CU).
+// The workgroup is gcu_q quarters of 256 threads, each quarter on its own
+// 64 x 64 tile of C, so that a CU has that many tiles in flight; a tile's A
+// (64 x K) and B (K x 64) are staged whole (one load round trip for the
+// products instead of one per GK slice), and the waves of the quarters hide
+// each other's latency. The quarters step through groups of tiles together
+// (every barrier is reached by all of them; a quarter past the last tile
+// loads and stores nothing).
+template <class T> constexpr int gcu_q() { return sizeof(T) == 4 ? 4 : 2; }
+template <class T> constexpr size_t gcu_quarter_lds() {
+  return (size_t)(2 * GT * (PW_ + 1) + 2 * PW_ * (GT + 1)) * sizeof(T);
+}
+template <class T> constexpr size_t gcu_lds() { return gcu_q<T>() * gcu_quarter_lds<T>(); }
+
+template <class T, bool ASSIGN>
+__global__ __launch_bounds__(gcu_q<T>() * 256) void k_gemm_cu(int M, int N, int K, const typename C2<T>::t *A,
+                                                               long lda, const typename C2<T>::t *B, long ldb,
+                                                               typename C2<T>::t *C, long ldc) {
+  typedef typename C2<T>::t cT;
+  typedef typename MF<T>::acc acc_t;
+  constexpr int Q = gcu_q<T>(), AS = PW_ + 1, BS = GT + 1;
+  extern __shared__ __align__(16) char smem_raw[];
+  const int sub = threadIdx.x >> 8, tid = threadIdx.x & 255, lane = tid & 63, wv = tid >> 6;
+  T *Ar = reinterpret_cast<T *>(smem_raw + sub * gcu_quarter_lds<T>());
+  T *Ai = Ar + GT * AS, *Br = Ai + GT * AS, *Bi = Br + PW_ * BS;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;   // wave quadrant of the tile
+  const int lr = lane & 15, lk = lane >> 4;
+  const int tn = (N + GT - 1) / GT, nt = ((M + GT - 1) / GT) * tn, ngrp = (nt + Q - 1) / Q;
+  for (int g = blockIdx.x; g < ngrp; g += gridDim.x) {
+    const int tile = g * Q + sub;
+    const bool valid = tile < nt;
+    const int m0 = valid ? (tile / tn) * GT : M, n0 = valid ? (tile % tn) * GT : N;
+    cT va[GT * PW_ / 256], vb[PW_ * GT / 256];
+#pragma unroll
+    for (int e = 0; e < GT * PW_ / 256; ++e) {
+      const int idx = tid + 256 * e;
+      const int ar = idx / PW_, ak = idx % PW_, bk = idx / GT, bn = idx % GT;
+      va[e] = (m0 + ar < M && ak < K) ? A[(long)(m0 + ar) * lda + ak] : cT{0, 0};
+      vb[e] = (bk < K && n0 + bn < N) ? B[(long)bk * ldb + n0 + bn] : cT{0, 0};
+    }
+    __syncthreads();   // the previous group's reads of the tiles are done
+#pragma unroll
+    for (int e = 0; e < GT * PW_ / 256; ++e) {
+      const int idx = tid + 256 * e;
+      const int ar = idx / PW_, ak = idx % PW_, bk = idx / GT, bn = idx % GT;
+      Ar[ar * AS + ak] = va[e].x;
+      Ai[ar * AS + ak] = va[e].y;
+      Br[bk * BS + bn] = vb[e].x;
+      Bi[bk * BS + bn] = vb[e].y;
+    }
+    __syncthreads();
+    acc_t pr[2][2], pi[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) { pr[a][b] = acc_t{0, 0, 0, 0}; pi[a][b] = acc_t{0, 0, 0, 0}; }
+#pragma unroll
+    for (int kk = 0; kk < PW_; kk += 4) {
+      if (kk >= K) break;   // uniform; entries past K are zero
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const T ar = Ar[(wm + a * 16 + lr) * AS + kk + lk], ai = Ai[(wm + a * 16 + lr) * AS + kk + lk];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const T br = Br[(kk + lk) * BS + wn + b * 16 + lr], bi = Bi[(kk + lk) * BS + wn + b * 16 + lr];
+          pr[a][b] = MF<T>::mma(ar, br, pr[a][b]);
+          pr[a][b] = MF<T>::mma(-ai, bi, pr[a][b]);
+          pi[a][b] = MF<T>::mma(ar, bi, pi[a][b]);
+          pi[a][b] = MF<T>::mma(ai, br, pi[a][b]);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = m0 + wm + a * 16 + MF<T>::row(lk, i), col = n0 + wn + b * 16 + lr;
+          if (row < M && col < N) {
+            if (ASSIGN) {
+              C[(long)row * ldc + col] = cT{pr[a][b][i], pi[a][b][i]};
+            } else {
+              const cT c = C[(long)row * ldc + col];
+              C[(long)row * ldc + col] = cT{c.x - pr[a][b][i], c.y - pi[a][b][i]};
+            }
+          }
+        }
+  }
+}
+
 // Strips that run beside the side stream's trailing update launch every
 // kernel with a dynamic-LDS request that fills the rest of a CU's 160 KB, so
 // that none of their workgroups shares a CU with a GEMM workgroup (they would
@@ -1257,8 +1350,8 @@ static const WholeCu<T> &whole_cu() {
     w.local = std::max(whole_cu_lds((const void *)k_ts_local<T>), ts_lds<T>());
     w.merge = std::max(whole_cu_lds((const void *)k_ts_merge<T>), ts_lds<T>());
     w.apply = whole_cu_lds((const void *)k_ts_apply<T>);
-    w.gemm_assign = whole_cu_lds((const void *)k_gemm_sub<T, true>);
-    w.gemm_sub = whole_cu_lds((const void *)k_gemm_sub<T, false>);
+    w.gemm_assign = std::max(whole_cu_lds((const void *)k_gemm_cu<T, true>), gcu_lds<T>());
+    w.gemm_sub = std::max(whole_cu_lds((const void *)k_gemm_cu<T, false>), gcu_lds<T>());
   });
   return w;
 }
@@ -1375,11 +1468,30 @@ static void getrf_batched(int n, int batch, typename C2<T>::t *A, long lda, long
       }
       const cT *linv = reinterpret_cast<const cT *>(fin + PanelWS<T>::fin_inv_off()), *uinv = linv + PW * PW;
       const int mr = n - c0 - w;
+      const int rest = k0 + nb - (c0 + w);
+      if (wcu) {   // single matrix, whole-CU launches (k_gemm_cu: several tiles per CU, K <= PW)
+        const dim3 bcu(gcu_q<T>() * 256);
+        auto gcu_grid = [](int M, int N) {
+          const int nt = ((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+          return dim3(std::min((nt + gcu_q<T>() - 1) / gcu_q<T>(), 4096));
+        };
+        if (mr > 0 && !pp)   // L21 = A21 U11^{-1}
+          QH_LAUNCH((k_gemm_cu<T, true>), gcu_grid(mr, w), bcu, l_asg, st, mr, w, w, A + (long)(c0 + w) * lda + c0,
+                    lda, uinv, (long)PW, A + (long)(c0 + w) * lda + c0, lda);
+        if (rest > 0) {   // the block's next columns: U = L11^{-1} A12, then A22 -= L21 U
+          QH_LAUNCH((k_gemm_cu<T, true>), gcu_grid(w, rest), bcu, l_asg, st, w, rest, w, linv, (long)PW,
+                    A + (long)c0 * lda + c0 + w, lda, A + (long)c0 * lda + c0 + w, lda);
+          if (mr > 0)
+            QH_LAUNCH((k_gemm_cu<T, false>), gcu_grid(mr, rest), bcu, l_sub, st, mr, rest, w,
+                      A + (long)(c0 + w) * lda + c0, lda, A + (long)c0 * lda + c0 + w, lda,
+                      A + (long)(c0 + w) * lda + c0 + w, lda);
+        }
+        continue;
+      }
       if (mr > 0 && !pp)   // L21 = A21 U11^{-1}
         QH_LAUNCH((k_gemm_sub<T, true>), dim3(1, (mr + GT - 1) / GT, batch), b256, l_asg, st, mr, w, w,
                   A + (long)(c0 + w) * lda + c0, lda, sA, uinv, (long)PW, sWc, A + (long)(c0 + w) * lda + c0, lda,
                   sA);
-      const int rest = k0 + nb - (c0 + w);
       if (rest > 0) {   // the block's next columns: U = L11^{-1} A12, then A22 -= L21 U
         QH_LAUNCH((k_gemm_sub<T, true>), dim3((rest + GT - 1) / GT, 1, batch), b256, l_asg, st, w, rest, w, linv,
                   (long)PW, sWc, A + (long)c0 * lda + c0 + w, lda, sA, A + (long)c0 * lda + c0 + w, lda, sA);

@@ -97,11 +97,13 @@ def test_zgetrf_zero_column_info(gpu, n, zc):
     assert info == infoo == zc + 1
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_getrf_batched(gpu, prec):
+# batch 1 above order 512: the lookahead path, whose strips run as whole-CU
+# launches (k_gemm_cu) beside the trailing update
+@pytest.mark.parametrize("prec,n,batch", [("f32", 700, 3), ("f64", 700, 3), ("f32", 1500, 1), ("f64", 1300, 1),
+                                          ("f32", 2081, 1)])
+def test_getrf_batched(gpu, prec, n, batch):
     import qpms_amd
     L = qpms_amd.lib()
-    n, batch = 700, 3
     dt = np.complex64 if prec == "f32" else np.complex128
     As = np.stack([rand(n, 100 + z, shift=3 * np.sqrt(n)) for z in range(batch)]).astype(dt)
     dA = qpms_amd.DeviceBuffer.from_array(As)
