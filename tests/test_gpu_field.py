@@ -30,7 +30,7 @@ def _case(rng, lMax, npart, npts):
 
 @pytest.mark.parametrize("lMax", [1, 2, 3, 4])
 @pytest.mark.parametrize("J", [1, 2, 3, 4])
-@pytest.mark.parametrize("npart", [5, 80])     # thread per point / wave per point
+@pytest.mark.parametrize("npart", [3, 5, 80])  # 3: one thread per point; 5, 80: particles split over 4 waves
 def test_scattered_field_vs_oracle(gpu, lMax, J, npart):
     import qpms_amd
     rng = np.random.default_rng(1000 * lMax + 10 * J + npart)
