@@ -130,6 +130,27 @@ def test_served_beyn_matches_oracle(gpu):
         assert ob.match_sets(g.eigenvalues(), ev) <= 1e-9
 
 
+def test_stream_large_requests_match_plain(gpu):
+    """Requests of the seeded serving stream at full size (orders up to
+    30,000: FP32 LU with lookahead and FP64 refinement, periodic
+    determinants) against the plain FP64 API on the same requests."""
+    from qpms_amd import serve as sv
+    stream = sv.serving_stream(n=400)
+    scat = sorted((r for r in stream if r.kind == sv.KIND_SCATTER and r.dim <= 30000), key=lambda r: -r.dim)[:2]
+    per = sorted((r for r in stream if r.kind == sv.KIND_PERIODIC and r.dim <= 12000), key=lambda r: -r.dim)[:1]
+    reqs = [r.clone(0.0) for r in scat + per]
+    assert reqs[0].dim > 20000 and per[0].dim > 6000, [r.dim for r in reqs]
+    print("orders", [r.dim for r in reqs], flush=True)
+    srv = sv.Server(arena_gib=64, nstreams=2, mixed=True)
+    ids = [srv.submit(r) for r in reqs]
+    got = drain(srv)
+    srv.close()
+    for r, i in zip(reqs, ids):
+        close(got[i], sv.request_plain(r), r.kind, 1e-9, r)
+        if r.kind == sv.KIND_SCATTER:
+            assert got[i].rel_residual < 1e-14
+
+
 def test_serving_fp64_mode_matches_plain(gpu):
     from qpms_amd import serve as sv
     reqs = [r for r in requests() if r.kind == sv.KIND_SCATTER][:6]
