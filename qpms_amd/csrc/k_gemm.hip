@@ -17,9 +17,11 @@
 // (v_mfma_f32_32x32x2f32). K advances BK = 16 complex per LDS stage; the
 // operands are de-interleaved into two planes each (A: Ar, Ai as [k][m]; B:
 // Br, Bi as [k][n], rows padded by one float; the sums Ar + Ai, Br + Bi are
-// formed in registers as the operands are read), double buffered, so the
-// global loads of the next stage are in flight while the current one is
-// multiplied, with one barrier per stage.
+// formed in registers as the operands are read), double buffered, with one
+// barrier per stage. When K is a multiple of BK (the LU's updates) the global
+// loads of a stage are issued two stages before it is multiplied, into one of
+// two register sets (unconditional loads, so each set's wait covers that set
+// only); otherwise one stage before.
 //
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2; the tile a workgroup takes is remapped so that every XCD works through
@@ -60,7 +62,7 @@ struct Tile {
   static_assert(BM % 64 == 0 && BN % 64 == 0 && THREADS % BM == 0 && BK % A_TPR == 0, "tile shape");
 };
 
-template <int BM, int BN>
+template <int BM, int BN, bool KF>
 __global__ __launch_bounds__(THREADS) void k_cgemm3m(int M, int N, int K, const v2f *__restrict__ A, long lda,
                                                      long sA, const v2f *__restrict__ B, long ldb, long sB,
                                                      v2f *__restrict__ C, long ldc, long sC) {
@@ -82,35 +84,50 @@ __global__ __launch_bounds__(THREADS) void k_cgemm3m(int M, int N, int K, const 
   const int ar = tid / T::A_TPR, ac = (tid % T::A_TPR) * T::A_PER;   // A: row, first k
   const int bk = tid / T::B_TPR, bc = (tid % T::B_TPR) * T::B_PER;   // B: k, first column
   const bool arow_ok = m0 + ar < M;
-  v2f ga[T::A_PER], gb[T::B_PER];
-  auto load = [&](int k0) {
+  // KF (K a multiple of BK): the operand loads are unconditional (a row of A
+  // or a column of B past the matrix reads the last one and reaches only
+  // entries of C that are not stored), so the wait before a register set's
+  // LDS store covers that set only, and two sets alternate: the loads of a
+  // stage are issued two stages before it is multiplied. Otherwise entries
+  // past the matrix are zero-filled and one set is used.
+  v2f ga[T::A_PER], gb[T::B_PER], ha[T::A_PER], hb[T::B_PER];
+  const long arow = (long)min(m0 + ar, M - 1) * lda;
+  auto load = [&](int k0, v2f *xa, v2f *xb) {
+    if (KF) {
+#pragma unroll
+      for (int c = 0; c < T::A_PER; ++c) xa[c] = A[arow + k0 + ac + c];
+      const long brow = (long)(k0 + bk) * ldb;
+#pragma unroll
+      for (int c = 0; c < T::B_PER; ++c) xb[c] = B[brow + min(n0 + bc + c, N - 1)];
+      return;
+    }
 #pragma unroll
     for (int c = 0; c < T::A_PER; ++c) {
       const int k = k0 + ac + c;
-      ga[c] = (arow_ok && k < K) ? A[(long)(m0 + ar) * lda + k] : v2f{0.f, 0.f};
+      xa[c] = (arow_ok && k < K) ? A[(long)(m0 + ar) * lda + k] : v2f{0.f, 0.f};
     }
     const bool bk_ok = k0 + bk < K;
 #pragma unroll
     for (int c = 0; c < T::B_PER; ++c) {
       const int n = n0 + bc + c;
-      gb[c] = (bk_ok && n < N) ? B[(long)(k0 + bk) * ldb + n] : v2f{0.f, 0.f};
+      xb[c] = (bk_ok && n < N) ? B[(long)(k0 + bk) * ldb + n] : v2f{0.f, 0.f};
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const v2f *xa, const v2f *xb) {
     float *s = lds + buf * T::STAGE;
     float *Ar = s, *Ai = s + T::PA;
     float *Br = s + 2 * T::PA, *Bi = Br + T::PB;
 #pragma unroll
     for (int c = 0; c < T::A_PER; ++c) {
       const int o = (ac + c) * T::SA + ar;
-      Ar[o] = ga[c].x;
-      Ai[o] = ga[c].y;
+      Ar[o] = xa[c].x;
+      Ai[o] = xa[c].y;
     }
 #pragma unroll
     for (int c = 0; c < T::B_PER; ++c) {
       const int o = bk * T::SB + bc + c;
-      Br[o] = gb[c].x;
-      Bi[o] = gb[c].y;
+      Br[o] = xb[c].x;
+      Bi[o] = xb[c].y;
     }
   };
 
@@ -122,14 +139,8 @@ __global__ __launch_bounds__(THREADS) void k_cgemm3m(int M, int N, int K, const 
 #pragma unroll
       for (int j = 0; j < T::TN; ++j) acc[p][i][j] = v16f{0};
 
-  load(0);
-  store(0);
-  __syncthreads();
-  int cur = 0;
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    const bool next = k0 + BK < K;
-    if (next) load(k0 + BK);
-    const float *s = lds + cur * T::STAGE;
+  auto multiply = [&](int buf) {
+    const float *s = lds + buf * T::STAGE;
     const float *Ar = s, *Ai = s + T::PA;
     const float *Br = s + 2 * T::PA, *Bi = Br + T::PB;
 #pragma unroll
@@ -156,9 +167,45 @@ __global__ __launch_bounds__(THREADS) void k_cgemm3m(int M, int N, int K, const 
           for (int j = 0; j < T::TN; ++j)
             acc[p][i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p][i], b[p][j], acc[p][i][j], 0, 0, 0);
     }
-    if (next) store(cur ^ 1);
+  };
+
+  if (KF) {
+    // stages in pairs, so that the register sets alternate without copies:
+    // at stage st the set g holds stage st + 1 and h receives st + 2. Loads
+    // past the last stage re-read it (straight-line code, no waits on
+    // conditions); what they store is never multiplied.
+    const int ns = K / BK;
+    load(0, ga, gb);
+    store(0, ga, gb);
+    load(min(1, ns - 1) * BK, ga, gb);
     __syncthreads();
-    cur ^= 1;
+    int st = 0;
+    for (; st + 1 < ns; st += 2) {
+      load(min(st + 2, ns - 1) * BK, ha, hb);
+      __builtin_amdgcn_sched_barrier(0);   // the loads stay ahead of the products
+      multiply(0);
+      store(1, ga, gb);
+      __syncthreads();
+      load(min(st + 3, ns - 1) * BK, ga, gb);
+      __builtin_amdgcn_sched_barrier(0);
+      multiply(1);
+      store(0, ha, hb);
+      __syncthreads();
+    }
+    if (st < ns) multiply(0);   // odd stage count: the last stage, stored by the loop
+  } else {
+    load(0, ga, gb);
+    store(0, ga, gb);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      const bool next = k0 + BK < K;
+      if (next) load(k0 + BK, ga, gb);
+      multiply(cur);
+      if (next) store(cur ^ 1, ga, gb);
+      __syncthreads();
+      cur ^= 1;
+    }
   }
 
 #pragma unroll
@@ -182,7 +229,9 @@ __global__ __launch_bounds__(THREADS) void k_cgemm3m(int M, int N, int K, const 
 template <int BM, int BN>
 void gemm_prepare() {
   static bool attr = [] {
-    HIPCHK(hipFuncSetAttribute((const void *)k_cgemm3m<BM, BN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_cgemm3m<BM, BN, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)Tile<BM, BN>::LDS));
+    HIPCHK(hipFuncSetAttribute((const void *)k_cgemm3m<BM, BN, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)Tile<BM, BN>::LDS));
     return true;
   }();
@@ -193,8 +242,14 @@ template <int BM, int BN>
 void gemm_launch(int M, int N, int K, int batch, const v2f *A, long lda, long sA, const v2f *B, long ldb, long sB,
             v2f *C, long ldc, long sC, hipStream_t s) {
   gemm_prepare<BM, BN>();
-  QH_LAUNCH((k_cgemm3m<BM, BN>), dim3((N + BN - 1) / BN, (M + BM - 1) / BM, batch), dim3(THREADS), (Tile<BM, BN>::LDS),
-            s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+  if (K <= 0) return;
+  const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, batch);
+  if (K % BK == 0)
+    QH_LAUNCH((k_cgemm3m<BM, BN, true>), grid, dim3(THREADS), (Tile<BM, BN>::LDS), s, M, N, K, A, lda, sA, B, ldb, sB,
+              C, ldc, sC);
+  else
+    QH_LAUNCH((k_cgemm3m<BM, BN, false>), grid, dim3(THREADS), (Tile<BM, BN>::LDS), s, M, N, K, A, lda, sA, B, ldb,
+              sB, C, ldc, sC);
 }
 
 }  // namespace

@@ -66,11 +66,13 @@ def test_gemm_sub_few_rhs_batched(gpu, N):
 
 @pytest.mark.parametrize("tile", [0, 1, 2])
 @pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (32, 32, 2, 1), (64, 64, 16, 2), (70, 45, 33, 1),
-                                         (130, 129, 64, 3), (257, 300, 256, 1), (5, 200, 7, 2), (400, 17, 300, 1)])
+                                         (130, 129, 64, 3), (257, 300, 256, 1), (5, 200, 7, 2), (400, 17, 300, 1),
+                                         (100, 90, 48, 2), (64, 64, 80, 1)])
 def test_cgemm3m_sub(gpu, tile, M, N, K, batch):
     """FP32 complex C -= A B by the three-product (3M) MFMA kernel, every tile
     shape, edges in M, N and K, batched with per-matrix strides and padded
-    leading dimensions."""
+    leading dimensions; K a multiple of 16 (the two-stage prefetch path) with
+    an even and an odd number of stages, and K that is not."""
     import qpms_amd
     L = qpms_amd.lib()
     rng = np.random.default_rng(7 * M + 11 * N + 13 * K + tile)
