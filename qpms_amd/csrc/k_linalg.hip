@@ -44,8 +44,6 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int NBW = 32;  // panel width
-
 template <class T> struct C2;
 template <> struct C2<double> { typedef v2d t; };
 template <> struct C2<float> { typedef v2f t; };
@@ -64,59 +62,6 @@ template <class T> __device__ __forceinline__ typename C2<T>::t cdiv(typename C2
   }
 }
 
-// A12 <- L11^{-1} A12 with L11 unit lower triangular (w x w at (k0, k0)).
-// Generic form: rows r0..r0+w of B (ldb) against the triangle at L (ldl).
-// One thread per column: the column's w entries are loaded before the
-// substitution and stored after it (stores between the loads would keep
-// each row's load waiting on the previous row's store).
-template <class T, bool UNIT_LOWER, int WM = NBW>
-__global__ __launch_bounds__(256) void k_trsm(int w, int ncols, const typename C2<T>::t *L, long ldl, long sL,
-                                              typename C2<T>::t *B, long ldb, long sB) {
-  typedef typename C2<T>::t cT;
-  constexpr int NBW = WM;
-  L += (long)blockIdx.z * sL;
-  B += (long)blockIdx.z * sB;
-  __shared__ cT sl[NBW * NBW];
-  for (int i = threadIdx.x; i < w * w; i += 256) sl[i] = L[(long)(i / w) * ldl + (i % w)];
-  __syncthreads();
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncols) return;
-  cT x[NBW];
-#pragma unroll
-  for (int r = 0; r < NBW; ++r) x[r] = (r < w) ? B[(long)r * ldb + c] : cT{0, 0};
-  if (UNIT_LOWER) {
-#pragma unroll
-    for (int r = 1; r < NBW; ++r) {
-      if (r < w) {
-        cT v = x[r];
-#pragma unroll
-        for (int s = 0; s < r; ++s) {
-          const cT l = sl[r * w + s];
-          v = cT{v.x - (l.x * x[s].x - l.y * x[s].y), v.y - (l.x * x[s].y + l.y * x[s].x)};
-        }
-        x[r] = v;
-      }
-    }
-  } else {  // upper, non-unit: backward substitution
-#pragma unroll
-    for (int r = NBW - 1; r >= 0; --r) {
-      if (r < w) {
-        cT v = x[r];
-#pragma unroll
-        for (int s = r + 1; s < NBW; ++s) {
-          if (s < w) {
-            const cT u = sl[r * w + s];
-            v = cT{v.x - (u.x * x[s].x - u.y * x[s].y), v.y - (u.x * x[s].y + u.y * x[s].x)};
-          }
-        }
-        x[r] = cdiv<T>(v, sl[r * w + r]);
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < NBW; ++r)
-    if (r < w) B[(long)r * ldb + c] = x[r];
-}
 
 // ---------------------------------------------------------------------------
 // C[M x N] -= A[M x K] * B[K x N] (complex, row-major), MFMA; with ASSIGN,
@@ -1371,9 +1316,12 @@ static void ts_prepare() {
 extern "C" void qh_cgemm3m_prepare(void);
 // Every kernel attribute the LU paths set, set now (the serving loop calls
 // this at creation, before it captures any launch sequence into a graph).
+template <class T> static void tsb_prepare();
 extern "C" void qh_lu_prepare(void) {
   ts_prepare<float>();
   ts_prepare<double>();
+  tsb_prepare<float>();
+  tsb_prepare<double>();
   qh_cgemm3m_prepare();
 }
 
@@ -1667,6 +1615,156 @@ __global__ __launch_bounds__(64) void k_trsv_update(int i0, int nrows, int r0, i
     }
 }
 
+// Many right-hand sides: the triangle in diagonal blocks of TsbRows rows
+// (256 FP32, 128 FP64), each solved by one launch of k_trsm_blk, and the rows
+// beyond the block updated by one GEMM with K = the block height: an eighth
+// (a quarter) of the dependent launches of 32-row blocks. k_trsm_blk takes a
+// slab of 32 right-hand-side columns per workgroup into LDS and walks the
+// block's 32-row sub-blocks: the sub-block's triangle and the panel of the
+// triangle beside it (the block's other rows, those columns) are staged in
+// LDS, every load issued before the substitution, which runs in registers,
+// one thread per column; then all threads subtract the panel times the
+// solved rows from the other rows, reading only LDS.
+template <class T> struct TsbRows;
+template <> struct TsbRows<float> { static constexpr int R = 256; };
+template <> struct TsbRows<double> { static constexpr int R = 128; };
+constexpr int TSB_SS = 33, TSB_THREADS = 512;   // LDS row stride (complex); workgroup size
+template <class T> constexpr size_t tsb_lds() {
+  return (size_t)(2 * TsbRows<T>::R * TSB_SS) * sizeof(typename C2<T>::t);   // slab + (sub-block and panel)
+}
+template <class T, bool LOWER>
+__global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, const typename C2<T>::t *L, long ldl,
+                                                          long sL, typename C2<T>::t *B, long ldb, long sB) {
+  typedef typename C2<T>::t cT;
+  constexpr int TB = TsbRows<T>::R, NPL = TB * 32 / TSB_THREADS;   // staged triangle entries per thread
+  extern __shared__ __align__(16) char smem_raw[];
+  cT *sb = reinterpret_cast<cT *>(smem_raw);   // [TB][TSB_SS] the slab
+  cT *sp = sb + TB * TSB_SS;                    // [TB][TSB_SS] rows of the triangle, columns of the sub-block
+  L += (long)blockIdx.z * sL;
+  B += (long)blockIdx.z * sB;
+  const int t = threadIdx.x, c0 = blockIdx.x * 32, nc = min(32, ncols - c0);
+  for (int e = t; e < w * 32; e += TSB_THREADS) {
+    const int r = e >> 5, c = e & 31;
+    sb[r * TSB_SS + c] = (c < nc) ? B[(long)r * ldb + c0 + c] : cT{0, 0};
+  }
+  const int nsub = (w + 31) / 32;
+  for (int q = 0; q < nsub; ++q) {
+    const int j0 = (LOWER ? q : nsub - 1 - q) * 32, jw = min(32, w - j0);
+    // columns j0..j0+jw of every row of the block (the sub-block's triangle
+    // and the panel), loads first
+    {
+      cT v[NPL];
+#pragma unroll
+      for (int u = 0; u < NPL; ++u) {
+        const int e = t + TSB_THREADS * u, r = e >> 5, k = e & 31;
+        v[u] = (r < w && k < jw) ? L[(long)r * ldl + j0 + k] : cT{0, 0};
+      }
+      __syncthreads();   // the previous sub-block's reads of sp are done
+#pragma unroll
+      for (int u = 0; u < NPL; ++u) {
+        const int e = t + TSB_THREADS * u;
+        sp[(e >> 5) * TSB_SS + (e & 31)] = v[u];
+      }
+    }
+    __syncthreads();
+    if (t < 32) {   // substitution on column t of the sub-block, in registers
+      cT x[32];
+#pragma unroll
+      for (int r = 0; r < 32; ++r) x[r] = (r < jw) ? sb[(j0 + r) * TSB_SS + t] : cT{0, 0};
+      // each row's sum in four interleaved partial sums (shorter dependent chains)
+      if (LOWER) {
+#pragma unroll
+        for (int r = 1; r < 32; ++r) {
+          if (r < jw) {
+            cT a[4] = {cT{0, 0}, cT{0, 0}, cT{0, 0}, cT{0, 0}};
+#pragma unroll
+            for (int k = 0; k < r; ++k) {
+              const cT l = sp[(j0 + r) * TSB_SS + k];
+              a[k & 3] = cT{a[k & 3].x + (l.x * x[k].x - l.y * x[k].y), a[k & 3].y + (l.x * x[k].y + l.y * x[k].x)};
+            }
+            x[r] = cT{x[r].x - ((a[0].x + a[1].x) + (a[2].x + a[3].x)), x[r].y - ((a[0].y + a[1].y) + (a[2].y + a[3].y))};
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 31; r >= 0; --r) {
+          if (r < jw) {
+            cT a[4] = {cT{0, 0}, cT{0, 0}, cT{0, 0}, cT{0, 0}};
+#pragma unroll
+            for (int k = r + 1; k < 32; ++k) {
+              const cT u = sp[(j0 + r) * TSB_SS + k];
+              a[k & 3] = cT{a[k & 3].x + (u.x * x[k].x - u.y * x[k].y), a[k & 3].y + (u.x * x[k].y + u.y * x[k].x)};
+            }
+            const cT v = cT{x[r].x - ((a[0].x + a[1].x) + (a[2].x + a[3].x)), x[r].y - ((a[0].y + a[1].y) + (a[2].y + a[3].y))};
+            x[r] = cdiv<T>(v, sp[(j0 + r) * TSB_SS + r]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 32; ++r)
+        if (r < jw) sb[(j0 + r) * TSB_SS + t] = x[r];
+    }
+    __syncthreads();
+    // the block's other rows: below the sub-block (L), above it (U)
+    const int u0 = LOWER ? j0 + jw : 0, u1 = LOWER ? w : j0, c = t & 31;
+    if (u1 > u0) {
+      cT x[32];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) x[k] = (k < jw) ? sb[(j0 + k) * TSB_SS + c] : cT{0, 0};
+      if constexpr (sizeof(T) == 4) {
+        // two rows per step, each in two partial sums
+        constexpr int RS = TSB_THREADS / 32;
+        for (int r = u0 + (t >> 5); r < u1; r += 2 * RS) {
+          const int r2 = r + RS;
+          const bool two = r2 < u1;
+          cT a0 = cT{0, 0}, a1 = cT{0, 0}, b0 = cT{0, 0}, b1 = cT{0, 0};
+#pragma unroll
+          for (int k = 0; k < 32; k += 2) {
+            const cT l0 = sp[r * TSB_SS + k], l1 = sp[r * TSB_SS + k + 1];
+            const cT m0 = two ? sp[r2 * TSB_SS + k] : cT{0, 0}, m1 = two ? sp[r2 * TSB_SS + k + 1] : cT{0, 0};
+            a0 = cT{a0.x + (l0.x * x[k].x - l0.y * x[k].y), a0.y + (l0.x * x[k].y + l0.y * x[k].x)};
+            a1 = cT{a1.x + (l1.x * x[k + 1].x - l1.y * x[k + 1].y), a1.y + (l1.x * x[k + 1].y + l1.y * x[k + 1].x)};
+            b0 = cT{b0.x + (m0.x * x[k].x - m0.y * x[k].y), b0.y + (m0.x * x[k].y + m0.y * x[k].x)};
+            b1 = cT{b1.x + (m1.x * x[k + 1].x - m1.y * x[k + 1].y), b1.y + (m1.x * x[k + 1].y + m1.y * x[k + 1].x)};
+          }
+          cT &y = sb[r * TSB_SS + c];
+          y = cT{y.x - (a0.x + a1.x), y.y - (a0.y + a1.y)};
+          if (two) {
+            cT &y2 = sb[r2 * TSB_SS + c];
+            y2 = cT{y2.x - (b0.x + b1.x), y2.y - (b0.y + b1.y)};
+          }
+        }
+      } else {   // FP64: one row per step (the two-row form exceeds the registers)
+        for (int r = u0 + (t >> 5); r < u1; r += TSB_THREADS / 32) {
+          cT acc = cT{0, 0};
+#pragma unroll
+          for (int k = 0; k < 32; ++k) {
+            const cT l = sp[r * TSB_SS + k];
+            acc = cT{acc.x + (l.x * x[k].x - l.y * x[k].y), acc.y + (l.x * x[k].y + l.y * x[k].x)};
+          }
+          cT &y = sb[r * TSB_SS + c];
+          y = cT{y.x - acc.x, y.y - acc.y};
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < w * 32; e += TSB_THREADS) {
+    const int r = e >> 5, c = e & 31;
+    if (c < nc) B[(long)r * ldb + c0 + c] = sb[r * TSB_SS + c];
+  }
+}
+template <class T>
+static void tsb_prepare() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_blk<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)tsb_lds<T>()));
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_blk<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)tsb_lds<T>()));
+  });
+}
+
 // B (n x nrhs, ldb) <- A^{-1} B given LU and the row permutation.
 template <class T>
 static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *LU, long lda, long sA,
@@ -1699,22 +1797,23 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
     HIPCHK(hipGetLastError());
     return;
   }
-  for (int r0 = 0; r0 < n; r0 += NBW) {
-    const int w = (n - r0 < NBW) ? n - r0 : NBW;
-    QH_LAUNCH((k_trsm<T, true>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
-                       LU + (long)r0 * lda + r0, lda, sA, work + (long)r0 * nrhs, (long)nrhs, sW);
+  tsb_prepare<T>();
+  const dim3 gs((nrhs + 31) / 32, 1, batch);
+  for (int r0 = 0; r0 < n; r0 += TsbRows<T>::R) {
+    const int w = min(TsbRows<T>::R, n - r0);
+    QH_LAUNCH((k_trsm_blk<T, true>), gs, dim3(TSB_THREADS), tsb_lds<T>(), s, w, nrhs, LU + (long)r0 * lda + r0, lda, sA,
+              work + (long)r0 * nrhs, (long)nrhs, sW);
     const int rem = n - r0 - w;
     if (rem > 0)
       QH_LAUNCH(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (rem + GT - 1) / GT, batch), b256, 0, s, rem, nrhs, w,
                          LU + (long)(r0 + w) * lda + r0, lda, sA, (const cT *)(work + (long)r0 * nrhs), (long)nrhs, sW,
                          work + (long)(r0 + w) * nrhs, (long)nrhs, sW);
   }
-  const int nblk = (n + NBW - 1) / NBW;
+  const int nblk = (n + TsbRows<T>::R - 1) / TsbRows<T>::R;
   for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int r0 = bi * NBW;
-    const int w = (n - r0 < NBW) ? n - r0 : NBW;
-    QH_LAUNCH((k_trsm<T, false>), dim3((nrhs + 255) / 256, 1, batch), b256, 0, s, w, nrhs,
-                       LU + (long)r0 * lda + r0, lda, sA, work + (long)r0 * nrhs, (long)nrhs, sW);
+    const int r0 = bi * TsbRows<T>::R, w = min(TsbRows<T>::R, n - r0);
+    QH_LAUNCH((k_trsm_blk<T, false>), gs, dim3(TSB_THREADS), tsb_lds<T>(), s, w, nrhs, LU + (long)r0 * lda + r0, lda, sA,
+              work + (long)r0 * nrhs, (long)nrhs, sW);
     if (r0 > 0)
       QH_LAUNCH(k_gemm_sub<T>, dim3((nrhs + GT - 1) / GT, (r0 + GT - 1) / GT, batch), b256, 0, s, r0, nrhs, w,
                          LU + r0, lda, sA, (const cT *)(work + (long)r0 * nrhs), (long)nrhs, sW, work, (long)nrhs, sW);
@@ -1975,9 +2074,11 @@ extern "C" int qh_gemm_sub_f64(int M, int N, int K, int batch, const double *A, 
     HIPCHK(hipGetLastError());
     return 0;
   }
-  QH_LAUNCH(k_gemm_sub<double>, dim3((N + GT - 1) / GT, (M + GT - 1) / GT, batch), dim3(256), 0,
-                     (hipStream_t)qh_stream(stream), M, N, K, (const v2d *)A, lda, sA, (const v2d *)B, ldb, sB,
-                     (v2d *)C, ldc, sC);
+  // more right-hand sides (the refinement residuals of Beyn nodes, N = l):
+  // the library GEMM outside graph capture (it streams A near the HBM rate;
+  // the tiled kernel reached a fifth of it at N = 32), the tiled kernel inside
+  trailing_gemm<double>(M, N, K, batch, (const v2d *)A, lda, sA, (const v2d *)B, ldb, sB, (v2d *)C, ldc, sC,
+                        (hipStream_t)qh_stream(stream));
   HIPCHK(hipGetLastError());
   return 0;
 }

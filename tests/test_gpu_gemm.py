@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (32, 3, 1), (1, 3, 32), (64, 64, 16), (70, 45, 33),
-                                   (130, 129, 64), (5, 200, 7)])
+                                   (130, 129, 64), (5, 200, 7), (9000, 32, 700)])   # last: the library path (FP64)
 def test_gemm_sub(gpu, prec, M, N, K):
     import qpms_amd
     L = qpms_amd.lib()

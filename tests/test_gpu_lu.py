@@ -129,3 +129,37 @@ def test_getrf_batched(gpu, prec, n, batch):
             j = piv[z][i] - 1
             p[[i, j]] = p[[j, i]]
         assert np.array_equal(perm[z], p)
+
+
+# many right-hand sides: 256-row diagonal blocks (k_trsm_blk) and K = 256
+# updates; orders that are not multiples of 256 or 32, slabs of 32 columns
+# with a partial last slab
+@pytest.mark.parametrize("prec,n,nrhs,batch", [("f32", 300, 5, 2), ("f32", 777, 32, 2), ("f64", 777, 40, 2),
+                                               ("f64", 2100, 32, 1), ("f32", 2100, 33, 1), ("f64", 255, 64, 3)])
+def test_getrs_batched_many_rhs(gpu, prec, n, nrhs, batch):
+    import qpms_amd
+    L = qpms_amd.lib()
+    dt = np.complex64 if prec == "f32" else np.complex128
+    As = np.stack([rand(n, 300 + z, shift=3 * np.sqrt(n)) for z in range(batch)]).astype(dt)
+    rng = np.random.default_rng(n + nrhs)
+    Bs = (rng.normal(size=(batch, n, nrhs)) + 1j * rng.normal(size=(batch, n, nrhs))).astype(dt)
+    dA = qpms_amd.DeviceBuffer.from_array(As)
+    dB = qpms_amd.DeviceBuffer.from_array(Bs)
+    dW = qpms_amd.DeviceBuffer(Bs.nbytes)
+    dpiv = qpms_amd.DeviceBuffer(4 * n * batch)
+    dperm = qpms_amd.DeviceBuffer(4 * n * batch)
+    dinfo = qpms_amd.DeviceBuffer.from_array(np.zeros(batch, np.int32))
+    i32, vp, lg = ctypes.c_int, ctypes.c_void_p, ctypes.c_long
+    f = L.qh_getrf_batched_f32 if prec == "f32" else L.qh_getrf_batched_f64
+    f.argtypes = [i32, i32, vp, lg, vp, vp, vp, vp]
+    f(n, batch, dA.ptr, n * n, dpiv.ptr, dperm.ptr, dinfo.ptr, None)
+    g = L.qh_getrs_batched_f32 if prec == "f32" else L.qh_getrs_batched_f64
+    g.argtypes = [i32, i32, i32, vp, lg, vp, vp, lg, vp, vp]
+    g(n, nrhs, batch, dA.ptr, n * n, dperm.ptr, dB.ptr, n * nrhs, dW.ptr, None)
+    L.qpms_hip_sync()
+    assert np.all(dinfo.to_array(np.int32, (batch,)) == 0)
+    X = dB.to_array(dt, (batch, n, nrhs)).astype(np.complex128)
+    tol = 2e-5 if prec == "f32" else 1e-12
+    for z in range(batch):
+        ref = np.linalg.solve(As[z].astype(np.complex128), Bs[z].astype(np.complex128))
+        assert np.max(np.abs(X[z] - ref)) / np.max(np.abs(ref)) < tol
