@@ -1628,9 +1628,10 @@ __global__ __launch_bounds__(64) void k_trsv_update(int i0, int nrows, int r0, i
 template <class T> struct TsbRows;
 template <> struct TsbRows<float> { static constexpr int R = 256; };
 template <> struct TsbRows<double> { static constexpr int R = 128; };
-constexpr int TSB_SS = 33, TSB_THREADS = 512;   // LDS row stride (complex); workgroup size
+constexpr int TSB_SS = 33, TSB_THREADS = 512;   // LDS row stride of the triangle's columns; workgroup size
+constexpr int TSB_SW = 8, TSB_BS = TSB_SW + 1;  // right-hand-side columns per workgroup; slab row stride
 template <class T> constexpr size_t tsb_lds() {
-  return (size_t)(2 * TsbRows<T>::R * TSB_SS) * sizeof(typename C2<T>::t);   // slab + (sub-block and panel)
+  return (size_t)(TsbRows<T>::R * (TSB_BS + TSB_SS)) * sizeof(typename C2<T>::t);   // slab + (sub-block and panel)
 }
 template <class T, bool LOWER>
 __global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, const typename C2<T>::t *L, long ldl,
@@ -1638,14 +1639,14 @@ __global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, cons
   typedef typename C2<T>::t cT;
   constexpr int TB = TsbRows<T>::R, NPL = TB * 32 / TSB_THREADS;   // staged triangle entries per thread
   extern __shared__ __align__(16) char smem_raw[];
-  cT *sb = reinterpret_cast<cT *>(smem_raw);   // [TB][TSB_SS] the slab
-  cT *sp = sb + TB * TSB_SS;                    // [TB][TSB_SS] rows of the triangle, columns of the sub-block
+  cT *sb = reinterpret_cast<cT *>(smem_raw);   // [TB][TSB_BS] the slab
+  cT *sp = sb + TB * TSB_BS;                    // [TB][TSB_SS] rows of the triangle, columns of the sub-block
   L += (long)blockIdx.z * sL;
   B += (long)blockIdx.z * sB;
-  const int t = threadIdx.x, c0 = blockIdx.x * 32, nc = min(32, ncols - c0);
-  for (int e = t; e < w * 32; e += TSB_THREADS) {
-    const int r = e >> 5, c = e & 31;
-    sb[r * TSB_SS + c] = (c < nc) ? B[(long)r * ldb + c0 + c] : cT{0, 0};
+  const int t = threadIdx.x, c0 = blockIdx.x * TSB_SW, nc = min(TSB_SW, ncols - c0);
+  for (int e = t; e < w * TSB_SW; e += TSB_THREADS) {
+    const int r = e / TSB_SW, c = e % TSB_SW;
+    sb[r * TSB_BS + c] = (c < nc) ? B[(long)r * ldb + c0 + c] : cT{0, 0};
   }
   const int nsub = (w + 31) / 32;
   for (int q = 0; q < nsub; ++q) {
@@ -1667,10 +1668,10 @@ __global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, cons
       }
     }
     __syncthreads();
-    if (t < 32) {   // substitution on column t of the sub-block, in registers
+    if (t < TSB_SW) {   // substitution on column t of the slab, in registers
       cT x[32];
 #pragma unroll
-      for (int r = 0; r < 32; ++r) x[r] = (r < jw) ? sb[(j0 + r) * TSB_SS + t] : cT{0, 0};
+      for (int r = 0; r < 32; ++r) x[r] = (r < jw) ? sb[(j0 + r) * TSB_BS + t] : cT{0, 0};
       // each row's sum in four interleaved partial sums (shorter dependent chains)
       if (LOWER) {
 #pragma unroll
@@ -1702,19 +1703,19 @@ __global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, cons
       }
 #pragma unroll
       for (int r = 0; r < 32; ++r)
-        if (r < jw) sb[(j0 + r) * TSB_SS + t] = x[r];
+        if (r < jw) sb[(j0 + r) * TSB_BS + t] = x[r];
     }
     __syncthreads();
     // the block's other rows: below the sub-block (L), above it (U)
-    const int u0 = LOWER ? j0 + jw : 0, u1 = LOWER ? w : j0, c = t & 31;
+    const int u0 = LOWER ? j0 + jw : 0, u1 = LOWER ? w : j0, c = t % TSB_SW;
     if (u1 > u0) {
       cT x[32];
 #pragma unroll
-      for (int k = 0; k < 32; ++k) x[k] = (k < jw) ? sb[(j0 + k) * TSB_SS + c] : cT{0, 0};
+      for (int k = 0; k < 32; ++k) x[k] = (k < jw) ? sb[(j0 + k) * TSB_BS + c] : cT{0, 0};
       if constexpr (sizeof(T) == 4) {
         // two rows per step, each in two partial sums
-        constexpr int RS = TSB_THREADS / 32;
-        for (int r = u0 + (t >> 5); r < u1; r += 2 * RS) {
+        constexpr int RS = TSB_THREADS / TSB_SW;
+        for (int r = u0 + t / TSB_SW; r < u1; r += 2 * RS) {
           const int r2 = r + RS;
           const bool two = r2 < u1;
           cT a0 = cT{0, 0}, a1 = cT{0, 0}, b0 = cT{0, 0}, b1 = cT{0, 0};
@@ -1727,31 +1728,31 @@ __global__ __launch_bounds__(TSB_THREADS) void k_trsm_blk(int w, int ncols, cons
             b0 = cT{b0.x + (m0.x * x[k].x - m0.y * x[k].y), b0.y + (m0.x * x[k].y + m0.y * x[k].x)};
             b1 = cT{b1.x + (m1.x * x[k + 1].x - m1.y * x[k + 1].y), b1.y + (m1.x * x[k + 1].y + m1.y * x[k + 1].x)};
           }
-          cT &y = sb[r * TSB_SS + c];
+          cT &y = sb[r * TSB_BS + c];
           y = cT{y.x - (a0.x + a1.x), y.y - (a0.y + a1.y)};
           if (two) {
-            cT &y2 = sb[r2 * TSB_SS + c];
+            cT &y2 = sb[r2 * TSB_BS + c];
             y2 = cT{y2.x - (b0.x + b1.x), y2.y - (b0.y + b1.y)};
           }
         }
       } else {   // FP64: one row per step (the two-row form exceeds the registers)
-        for (int r = u0 + (t >> 5); r < u1; r += TSB_THREADS / 32) {
+        for (int r = u0 + t / TSB_SW; r < u1; r += TSB_THREADS / TSB_SW) {
           cT acc = cT{0, 0};
 #pragma unroll
           for (int k = 0; k < 32; ++k) {
             const cT l = sp[r * TSB_SS + k];
             acc = cT{acc.x + (l.x * x[k].x - l.y * x[k].y), acc.y + (l.x * x[k].y + l.y * x[k].x)};
           }
-          cT &y = sb[r * TSB_SS + c];
+          cT &y = sb[r * TSB_BS + c];
           y = cT{y.x - acc.x, y.y - acc.y};
         }
       }
     }
   }
   __syncthreads();
-  for (int e = t; e < w * 32; e += TSB_THREADS) {
-    const int r = e >> 5, c = e & 31;
-    if (c < nc) B[(long)r * ldb + c0 + c] = sb[r * TSB_SS + c];
+  for (int e = t; e < w * TSB_SW; e += TSB_THREADS) {
+    const int r = e / TSB_SW, c = e % TSB_SW;
+    if (c < nc) B[(long)r * ldb + c0 + c] = sb[r * TSB_BS + c];
   }
 }
 template <class T>
@@ -1798,7 +1799,7 @@ static void getrs_batched(int n, int nrhs, int batch, const typename C2<T>::t *L
     return;
   }
   tsb_prepare<T>();
-  const dim3 gs((nrhs + 31) / 32, 1, batch);
+  const dim3 gs((nrhs + TSB_SW - 1) / TSB_SW, 1, batch);
   for (int r0 = 0; r0 < n; r0 += TsbRows<T>::R) {
     const int w = min(TsbRows<T>::R, n - r0);
     QH_LAUNCH((k_trsm_blk<T, true>), gs, dim3(TSB_THREADS), tsb_lds<T>(), s, w, nrhs, LU + (long)r0 * lda + r0, lda, sA,
