@@ -71,6 +71,11 @@ int qh_getrf_batched_wsla_f64(int n, int batch, double *A, long sA, int *ipiv, i
  * 96-node contour of order 9216 took 4.7 s captured against 4.0 s eager). */
 #define SRV_LOOKAHEAD_MIN 4096
 #define SRV_GRAPH_MAX 4096
+/* Beyn contours of this order and above take one node per chunk: a single
+ * LU with the lookahead beats a batch without it there (two LUs of order
+ * 38100: 2.61 s singly against 2.70 s as a batch of two; at order 9210 the
+ * batch wins, 101 against 146 ms). */
+#define SRV_BEYN_SINGLE_MIN 16384
 int qh_beyn_accum(long n, const double *X, double dzr, double dzi, double wr, double wi, int coarse,
 		double *A0, double *A1, double *A0c, double *A1c, void *stream);
 int qh_beyn_process_s(size_t m, size_t l, const complex double *dA0, const complex double *dA1, complex double z0,
@@ -411,10 +416,11 @@ static size_t job_bytes(const qpms_hip_server_t *s, const qh_job *j, int *beyn_c
 			const size_t l = (size_t)j->req.beyn_l;
 			int c = j->req.beyn_nodes;
 			const int N = j->req.beyn_nodes, lm = j->req.lMax;
-			/* the largest divisor of N that fits half the arena: every chunk then has
-			 * the same launch sequence (one captured graph serves them all) */
+			/* the largest divisor of N that fits half the arena (one node from
+			 * SRV_BEYN_SINGLE_MIN on): every chunk then has the same launch
+			 * sequence (one captured graph serves them all) */
 			while (c > 1 && (N % c || beyn_bytes(s, L, l, c, N, lm) > s->arena.size / 2 ||
-					(s->beyn_chunk_max > 0 && c > s->beyn_chunk_max))) --c;
+					(s->beyn_chunk_max > 0 && c > s->beyn_chunk_max) || L >= SRV_BEYN_SINGLE_MIN)) --c;
 			if (beyn_chunk) *beyn_chunk = c;
 			return beyn_bytes(s, L, l, c, N, lm);
 		}
